@@ -1,0 +1,150 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X MPS particle time step.
+
+Workload (default): the 3-D dam break scaled to 1,049,600 fluid particles
+(80 x 80 x 164 lattice, l0 = 2.5 mm) in an open-top tank of 120 x 80 x 170
+inner lattice sites lined with 2 wall + 2 dummy layers: N = 1,377,536
+particles.  re_n = 2.1 l0, re_lap = 4.0 l0, rho = 1000, nu = 1e-6,
+g = -9.81 z, c = 20 m/s, dt_max = 2.5e-5 s (acoustic Courant number 0.2 as
+at l0 = 10 mm with dt = 1e-4 s), FP32 state with FP64 n and (n - n0)/n0.
+A step is one explicit MPS step (re-binning, viscosity, collision, PND +
+pressure, pressure gradient + update) over the resident state.
+
+Prints ONE JSON line: value = fluid particle-updates/s over the timed steps
+(inputs resident in HBM), roofline of the dominant kernel from HIP events on
+the library's stream, and the CPU oracle timed on this host on a bounded
+sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per particle per launch (FP32 pos/vel 12 B, FP64 n 8 B,
+# FP32 p 4 B, type/id/key/perm 4 B; neighbour gathers counted as cache hits)
+ALG_BYTES = {"visc": 52, "collision": 52, "pnd_pres": 28, "gradp": 56, "sort": 100}
+STEP_BYTES = 288
+
+WORKLOADS = {
+    1: dict(name="dam break 1.05M fluid (80x80x164, l0=2.5mm), tank 120x80x170 inner, 2+2 wall layers",
+            fluid=(80, 80, 164), inner=(120, 80, 170), dp=0.0025, dt_max=2.5e-5, max_dis=0.00025),
+    0: dict(name="dam break 16k fluid (20x20x40, l0=10mm), tank 80x20x50 inner, 2+2 wall layers",
+            fluid=(20, 20, 40), inner=(80, 20, 50), dp=0.01, dt_max=1e-4, max_dis=0.001),
+}
+
+
+def params_for(mps, w, precision=0):
+    return mps.params(dim=3, precision=precision, rho=1000.0, nu=1e-6, dp=w["dp"], re_n_mult=2.1,
+                      re_lap_mult=4.0, beta=0.97, cs=20.0, cfl=0.1, col_mult=0.8, col_rate=2.0,
+                      dt_max=w["dt_max"], dt_min_mult=1e-3, max_dis=w["max_dis"],
+                      g=(0.0, 0.0, -9.81))
+
+
+def cpu_baseline(w, nb2, nb1, pos, budget_s=20.0):
+    """Oracle (C/OpenMP restatement of the reference CPU path) on this host's
+    cores, on a bounded sample: whole explicit steps of the same scene."""
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    P = oracle.params(dim=3, f32=1, rho=1000.0, nu=1e-6, dp=w["dp"], re_n_mult=2.1,
+                      re_lap_mult=4.0, beta=0.97, cs=20.0, cfl=0.1, col_mult=0.8, col_rate=2.0,
+                      dt_max=w["dt_max"], dt_min_mult=1e-3, max_dis=w["max_dis"])
+    sim = oracle.Sim(P, nb2, nb1, pos[:, 0], pos[:, 1], pos[:, 2])
+    nfl = len(pos) - nb2 - nb1
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        sim.step(1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or steps >= 50:
+            break
+    return {"value": nfl * steps / el, "unit": "particle-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{steps} explicit step(s) of the same scene after initialisation, "
+                      f"{el:.1f} s on {threads} OpenMP threads (oracle/mps_oracle.c, FP64 sums)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--config", type=int, default=1, choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the explicit "
+              "step would shard by spatial slabs, see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import mps_solvers_amd as mps
+
+    w = WORKLOADS[a.config]
+    nb2, nb1, pos = mps.dambreak(w["fluid"], w["inner"], 2, 2, w["dp"], 0.01, 0)
+    N = len(pos)
+    nfl = N - nb2 - nb1
+    print(f"[bench] scene N={N} fluid={nfl} walls+dummies={nb2 + nb1}", file=sys.stderr, flush=True)
+    s = mps.Solver(params_for(mps, w), nb2, nb1, pos)
+    s.step(a.warmup)
+    t0 = time.perf_counter()
+    s.step(a.steps)
+    el = time.perf_counter() - t0
+    ms_step = 1e3 * el / max(a.steps, 1)
+    print(f"[bench] {a.steps} steps in {el:.3f} s ({ms_step:.3f} ms/step)", file=sys.stderr, flush=True)
+
+    # per-stage device time from HIP events on the library's stream
+    nt = min(20, max(a.steps, 1))
+    s.enable_timing(True)
+    s.stage_times(reset=True)
+    s.step(nt)
+    st = s.stage_times(reset=True)
+    s.enable_timing(False)
+    per = {k: v / nt for k, v in st.items()}
+    dom = max(("visc", "collision", "pnd_pres", "gradp", "sort"), key=lambda k: per[k])
+    ach = ALG_BYTES[dom] * N / (per[dom] * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(HERE, "profiles", f"traffic_{dom}_config{a.config}.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("bytes_per_launch")
+    sc = s.scalars()
+    out = {
+        "metric": "particle-updates/s at 1M particles; PPE ms/solve to 1e-8 residual, learned vs plain",
+        "value": nfl * a.steps / el,
+        "unit": "particle-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": w["name"], "n_total": N, "n_fluid": nfl, "l0": w["dp"],
+                   "dt": sc["dt"], "re_n": 2.1, "re_lap": 4.0,
+                   "all_particle_updates_per_s": N * a.steps / el,
+                   "step_roofline_frac": STEP_BYTES * N / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                   "stage_ms": {k: round(v, 4) for k, v in per.items()}},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic},
+    }
+    s.close()
+    if not a.no_cpu:
+        print("[bench] timing the CPU oracle on a bounded sample", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(w, nb2, nb1, pos, a.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

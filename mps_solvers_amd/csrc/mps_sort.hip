@@ -1,0 +1,242 @@
+// mps_sort.hip -- cell hash, LDS-staged stable LSD radix sort, cell bounds and
+// the reorder gather: the GPU form of MPS_GPU::divideCell + sort_* +
+// sort_normal (MPS_CAL_expl_v1.0/MPS_GPU.cpp:248-373, MPS_CAL_/MPS_GPU.cpp:811-895).
+//
+// The reference hands out slots with a serial counting sort that is stable in
+// the current order.  A stable LSD radix sort over the cell key with the
+// current index as payload yields exactly that permutation.
+#include "mps_device.hpp"
+#include "mps_kernels.hpp"
+
+namespace mps {
+
+// ---------------------------------------------------------------- hash ---
+// c_a = integer((pos_a - origin_a) / h), truncation toward zero, taken in the
+// state precision (MPS_CAL_expl_v1.0/MPS_GPU.cpp:265-268).  Out-of-grid
+// particles become type 3 with zero velocity (MPS_GPU.cpp:270-280) and get
+// key = ncells so that they sort after every cell.
+template <typename Real>
+__global__ void __launch_bounds__(256) k_hash(Grid g, int n, const Real* __restrict__ x,
+                                              const Real* __restrict__ y, const Real* __restrict__ z,
+                                              int* __restrict__ type, Real* u, Real* v, Real* w,
+                                              uint32_t* __restrict__ key, uint32_t* __restrict__ val,
+                                              int* __restrict__ nexcl)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    val[i] = (uint32_t)i;
+    int t = type[i];
+    if (t == 3) { key[i] = (uint32_t)g.ncells; return; }
+    const Real left = (Real)g.left, back = (Real)g.back, bottom = (Real)g.bottom, h = (Real)g.h;
+    Real fx = (x[i] - left) / h;
+    Real fy = (y[i] - back) / h;
+    Real fz = (z[i] - bottom) / h;
+    Real tx = trunc(fx), ty = trunc(fy), tz = trunc(fz);
+    bool in = tx >= (Real)0 && ty >= (Real)0 && tz >= (Real)0 && tx < (Real)g.dx &&
+              ty < (Real)g.dy && tz < (Real)g.dz;
+    if (!in) {
+        key[i] = (uint32_t)g.ncells;
+        type[i] = 3;
+        u[i] = (Real)0; v[i] = (Real)0; w[i] = (Real)0;
+        atomicAdd(nexcl, 1);
+        return;
+    }
+    key[i] = (uint32_t)((int)tz * g.sheet + (int)ty * g.dx + (int)tx);
+}
+
+// ------------------------------------------------------------ radix sort ---
+constexpr int RS_THREADS = 256;
+constexpr int RS_ITEMS = 8;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+
+// Per-block digit histogram.  Keys arrive nearly sorted (one re-binning per
+// step moves few particles), so equal digits come in runs: each run head adds
+// the run length once instead of one LDS atomic per key.
+__global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const uint32_t* __restrict__ keys, int n,
+                                                           int shift, uint32_t* __restrict__ hist)
+{
+    __shared__ uint32_t h[256];
+    const int tid = threadIdx.x, lane = tid & 63;
+    h[tid] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ITEMS; r++) {
+        int i = base + r * RS_THREADS + tid;
+        bool valid = i < n;
+        uint32_t d = valid ? (keys[i] >> shift) & 255u : 0u;
+        uint32_t dp = __shfl_up(d, 1);
+        bool head = valid && (lane == 0 || dp != d);
+        uint64_t bound = ballot(head || !valid);
+        if (head) {
+            uint64_t up = lane == 63 ? 0ull : (bound >> (lane + 1));
+            int next = up ? lane + 1 + __builtin_ctzll(up) : 64;
+            atomicAdd(&h[d], (uint32_t)(next - lane));
+        }
+    }
+    __syncthreads();
+    hist[(size_t)tid * gridDim.x + blockIdx.x] = h[tid];
+}
+
+// In-place exclusive scan of m counters by one 1024-thread block.
+__global__ void __launch_bounds__(1024) k_scan_excl(uint32_t* __restrict__ a, int m)
+{
+    __shared__ uint32_t part[1024];
+    const int tid = threadIdx.x;
+    const int per = (m + 1023) / 1024;
+    const int s0 = min(tid * per, m), s1 = min(s0 + per, m);
+    uint32_t s = 0;
+    for (int i = s0; i < s1; i++) s += a[i];
+    part[tid] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t t = tid >= off ? part[tid - off] : 0u;
+        __syncthreads();
+        part[tid] += t;
+        __syncthreads();
+    }
+    uint32_t run = part[tid] - s;
+    for (int i = s0; i < s1; i++) { uint32_t t = a[i]; a[i] = run; run += t; }
+}
+
+// Stable scatter.  Each round covers 256 consecutive keys; within a wave the
+// lanes holding the same digit are found with 8 ballots, ranks follow lane
+// order, and per-wave digit counts in LDS order the 4 waves.  Rounds advance a
+// per-digit running offset, so the output order is the input order within
+// each digit: the pass is stable.
+__global__ void __launch_bounds__(RS_THREADS) k_radix_scatter(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
+    uint32_t* __restrict__ vout, int n, int shift, const uint32_t* __restrict__ offs)
+{
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wc[4][256];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    run[tid] = offs[(size_t)tid * gridDim.x + blockIdx.x];
+    wc[0][tid] = 0; wc[1][tid] = 0; wc[2][tid] = 0; wc[3][tid] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ITEMS; r++) {
+        int i = base + r * RS_THREADS + tid;
+        bool valid = i < n;
+        uint32_t k = valid ? kin[i] : 0u;
+        uint32_t v = valid ? vin[i] : 0u;
+        uint32_t d = (k >> shift) & 255u;
+        uint64_t m = ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            bool bit = (d >> b) & 1u;
+            uint64_t bb = ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+        int rank = lanes_below(m);
+        if (valid && rank == 0) wc[wv][d] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (valid) {
+            uint32_t pre = run[d];
+            for (int q = 0; q < wv; q++) pre += wc[q][d];
+            uint32_t dst = pre + (uint32_t)rank;
+            kout[dst] = k;
+            vout[dst] = v;
+        }
+        __syncthreads();
+        run[tid] += wc[0][tid] + wc[1][tid] + wc[2][tid] + wc[3][tid];
+        wc[0][tid] = 0; wc[1][tid] = 0; wc[2][tid] = 0; wc[3][tid] = 0;
+        __syncthreads();
+    }
+}
+
+// cell_start[c] = first sorted slot whose key >= c (the reference's exclusive
+// scan of per-cell counts, MPS_GPU.cpp:293-298); cell_start[ncells] = number
+// of particles still inside the grid.
+__global__ void __launch_bounds__(256) k_cell_bounds(const uint32_t* __restrict__ keys, int n,
+                                                     int ncells, int* __restrict__ cs)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    int k = i < n ? (int)keys[i] : ncells + 1;
+    int kp = i == 0 ? -1 : (int)keys[i - 1];
+    int hi = min(k, ncells);
+    for (int c = kp + 1; c <= hi; c++) cs[c] = i;
+}
+
+// ------------------------------------------------------------- reorder ---
+// Gather every per-particle array into sorted order (sort_i/d/d3,
+// MPS_GPU.cpp:375-566) and record the inverse permutation for sort_normal.
+template <typename Real>
+__global__ void __launch_bounds__(256) k_reorder(int n, const uint32_t* __restrict__ perm,
+                                                 StateView<Real> src, StateView<Real> dst,
+                                                 int* __restrict__ inv, int* __restrict__ normal_tmp)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int o = (int)perm[k];
+    dst.x[k] = src.x[o]; dst.y[k] = src.y[o]; dst.z[k] = src.z[o];
+    dst.u[k] = src.u[o]; dst.v[k] = src.v[o]; dst.w[k] = src.w[o];
+    dst.p[k] = src.p[o]; dst.n[k] = src.n[o];
+    dst.type[k] = src.type[o]; dst.id[k] = src.id[o];
+    normal_tmp[k] = src.normal[o];
+    inv[o] = k;
+}
+
+// i_normal = i_index[i_normal] (MPS_GPU.cpp:336-373)
+__global__ void __launch_bounds__(256) k_normal_remap(int n, const int* __restrict__ normal_tmp,
+                                                      const int* __restrict__ inv, int* __restrict__ normal)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int o = normal_tmp[k];
+    normal[k] = o >= 0 ? inv[o] : -1;
+}
+
+// ------------------------------------------------------------- drivers ---
+static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+int sort_scratch_words(int n) { return 256 * cdiv(n, RS_TILE); }
+
+template <typename Real>
+void launch_hash(const Grid& g, int n, StateView<Real> s, uint32_t* key, uint32_t* val, int* nexcl,
+                 hipStream_t st)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_hash<Real>, dim3(cdiv(n, 256)), dim3(256), 0, st, g, n, s.x, s.y, s.z,
+                       s.type, s.u, s.v, s.w, key, val, nexcl);
+}
+
+void launch_radix_sort(int n, int key_bits, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
+                       uint32_t* hist, uint32_t** kres, uint32_t** vres, hipStream_t st)
+{
+    *kres = k0; *vres = v0;
+    if (n == 0) return;
+    const int nb = cdiv(n, RS_TILE);
+    for (int shift = 0; shift < key_bits; shift += 8) {
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(RS_THREADS), 0, st, k0, n, shift, hist);
+        hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, hist, 256 * nb);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(RS_THREADS), 0, st, k0, v0, k1, v1, n,
+                           shift, hist);
+        uint32_t* t = k0; k0 = k1; k1 = t;
+        t = v0; v0 = v1; v1 = t;
+    }
+    *kres = k0; *vres = v0;
+}
+
+void launch_cell_bounds(const uint32_t* keys, int n, int ncells, int* cs, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_cell_bounds, dim3(cdiv(n + 1, 256)), dim3(256), 0, st, keys, n, ncells, cs);
+}
+
+template <typename Real>
+void launch_reorder(int n, const uint32_t* perm, StateView<Real> src, StateView<Real> dst, int* inv,
+                    int* normal_tmp, hipStream_t st)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_reorder<Real>, dim3(cdiv(n, 256)), dim3(256), 0, st, n, perm, src, dst, inv,
+                       normal_tmp);
+    hipLaunchKernelGGL(k_normal_remap, dim3(cdiv(n, 256)), dim3(256), 0, st, n, normal_tmp, inv,
+                       dst.normal);
+}
+
+template void launch_hash<float>(const Grid&, int, StateView<float>, uint32_t*, uint32_t*, int*, hipStream_t);
+template void launch_hash<double>(const Grid&, int, StateView<double>, uint32_t*, uint32_t*, int*, hipStream_t);
+template void launch_reorder<float>(int, const uint32_t*, StateView<float>, StateView<float>, int*, int*, hipStream_t);
+template void launch_reorder<double>(int, const uint32_t*, StateView<double>, StateView<double>, int*, int*, hipStream_t);
+
+}  // namespace mps

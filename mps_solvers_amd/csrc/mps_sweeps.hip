@@ -1,0 +1,681 @@
+// mps_sweeps.hip -- the neighbour sweeps of the MPS step on gfx950.
+//
+// One 64-lane wavefront per cell (in chunks of 64 particles).  For each chunk
+// the wave takes the bounding box of the particles the operator updates,
+// streams the 27-cell stencil (9 contiguous rows of the cell-sorted arrays)
+// with coalesced loads, keeps only candidates closer to the box than the
+// operator's radius, compacts them into LDS with a ballot prefix, and then
+// broadcasts each staged candidate to all 64 lanes.  Every particle's sum
+// therefore sees exactly the reference's within-cutoff neighbour set.
+//
+// Operators (reference semantics, two-pass as in MPS_CAL_expl_v1.0):
+//   PND + explicit pressure  MPS_GPU::cal_n (MPS_GPU.cpp:830-887) + calPres_expl
+//                            (MPS_GPU_EXPL.cpp:199-238): n in FP64, p = max(0, rho c^2 (n-n0)/n0)
+//   viscosity + gravity      calVisc_expl (MPS_GPU_EXPL.cpp:152-194), d3_LapVel (MPS_GPU.cpp:683-717)
+//   collision                MPS_GPU::collision (MPS_GPU.cpp:892-986)
+//   pressure gradient        MPS_GPU::calDash (MPS_GPU.cpp:782-825), d3_GradPres (MPS_GPU.cpp:571-638)
+#include <float.h>
+
+#include "mps_device.hpp"
+#include "mps_kernels.hpp"
+
+namespace mps {
+
+template <typename Real>
+struct Cand {
+    Real x, y, z;
+    int j, t;
+    Real a, b, c;
+};
+
+template <typename Real> struct Cap { static constexpr int v = 512; };
+template <> struct Cap<double> { static constexpr int v = 256; };
+
+template <typename Real> __device__ __forceinline__ Real big();
+template <> __device__ __forceinline__ float big<float>() { return FLT_MAX; }
+template <> __device__ __forceinline__ double big<double>() { return DBL_MAX; }
+
+template <typename Real>
+__device__ __forceinline__ Real box_d2(Real qx, Real qy, Real qz, const Real* lo, const Real* hi)
+{
+    Real dx = fmax(fmax(lo[0] - qx, qx - hi[0]), (Real)0);
+    Real dy = fmax(fmax(lo[1] - qy, qy - hi[1]), (Real)0);
+    Real dz = fmax(fmax(lo[2] - qz, qz - hi[2]), (Real)0);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+struct StageResult { int count; bool overflowed; };
+
+// Stream the stencil of `cell`, keep candidates within sqrt(R2) of the box,
+// and hand full LDS batches to op.process().  The last batch stays in `buf`.
+template <typename Real, class Op>
+__device__ StageResult stage(const Grid& g, const int* __restrict__ cs, int cell, const Real* lo,
+                             const Real* hi, Real R2, Cand<Real>* buf, Op& op)
+{
+    constexpr int CAP = Cap<Real>::v;
+    const int lane = threadIdx.x;
+    int cx, cy, cz;
+    cell_xyz(g, cell, cx, cy, cz);
+    const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < g.dx ? cx + 1 : g.dx - 1;
+    int count = 0;
+    bool over = false;
+    for (int dz = -1; dz <= 1; dz++) {
+        const int zz = cz + dz;
+        if (zz < 0 || zz >= g.dz) continue;
+        for (int dy = -1; dy <= 1; dy++) {
+            const int yy = cy + dy;
+            if (yy < 0 || yy >= g.dy) continue;
+            const int row = zz * g.sheet + yy * g.dx;
+            const int a = cs[row + x0], b = cs[row + x1 + 1];
+            for (int base = a; base < b; base += WAVE) {
+                const int j = base + lane;
+                Cand<Real> c;
+                bool keep = false;
+                if (j < b) {
+                    op.load_cand(j, c);
+                    keep = box_d2(c.x, c.y, c.z, lo, hi) < R2;
+                }
+                const uint64_t m = ballot(keep);
+                if (keep) buf[count + lanes_below(m)] = c;
+                count += __popcll(m);
+                if (count > CAP - WAVE) {
+                    __syncthreads();
+                    op.process(buf, count);
+                    __syncthreads();
+                    count = 0;
+                    over = true;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (count) op.process(buf, count);
+    return {count, over};
+}
+
+// Bounding box of the active lanes, widened by one part in 1e5 so that the
+// staging filter never drops a candidate the exact tests would keep.
+template <typename Real>
+__device__ __forceinline__ void wave_box(bool act, Real x, Real y, Real z, Real* lo, Real* hi)
+{
+    lo[0] = wave_min(act ? x : big<Real>());
+    lo[1] = wave_min(act ? y : big<Real>());
+    lo[2] = wave_min(act ? z : big<Real>());
+    hi[0] = wave_max(act ? x : -big<Real>());
+    hi[1] = wave_max(act ? y : -big<Real>());
+    hi[2] = wave_max(act ? z : -big<Real>());
+}
+
+template <typename Real>
+__device__ __forceinline__ Real filt_r2(Real R) { Real r = R * (Real)1.00001 + (Real)1e-12; return r * r; }
+
+// ------------------------------------------------------------------ PND ---
+template <typename Real>
+struct OpPnd {
+    const Real *x, *y, *z;
+    const int* type;
+    Real re;     // R0 in the state precision
+    Real tn;     // rr < tn  <=>  sqrt(rr) < R0
+    int all_types;
+    int i;
+    bool active;
+    Real xi, yi, zi;
+    double acc;
+
+    __device__ void load_self(int i_, bool in)
+    {
+        i = i_;
+        acc = 0.0;
+        active = false;
+        if (in) {
+            xi = x[i]; yi = y[i]; zi = z[i];
+            active = all_types || type[i] != 2;
+        }
+    }
+    __device__ void load_cand(int j, Cand<Real>& c) { c.x = x[j]; c.y = y[j]; c.z = z[j]; c.j = j; }
+    __device__ void process(const Cand<Real>* buf, int cnt)
+    {
+        const double red = (double)re;
+        for (int k = 0; k < cnt; k++) {
+            const Real cxk = buf[k].x, cyk = buf[k].y, czk = buf[k].z;
+            const int cj = buf[k].j;
+            const Real dx = cxk - xi, dy = cyk - yi, dz = czk - zi;
+            const Real rr = rr3(dx, dy, dz);
+            if (rr < tn && cj != i) {
+                const double ddx = dx, ddy = dy, ddz = dz;
+                const double r2 = __builtin_fma(ddx, ddx, __builtin_fma(ddy, ddy, ddz * ddz));
+                acc += __builtin_fma(red, rsq(r2), -1.0);
+            }
+        }
+    }
+};
+
+template <typename Real>
+__global__ void __launch_bounds__(64) k_pnd(Grid g, const int* __restrict__ cs, StateView<Real> s,
+                                            Thresh<Real> th, const DevScalars* __restrict__ sc,
+                                            double one_over_alpha, int all_types, int with_pressure)
+{
+    __shared__ Cand<Real> buf[Cap<Real>::v];
+    const int cell = xcd_cell(g.ncells);
+    if (cell >= g.ncells) return;
+    const int s0 = cs[cell], s1 = cs[cell + 1];
+    const double n0 = with_pressure ? sc->n0 : 1.0;
+    OpPnd<Real> op;
+    op.x = s.x; op.y = s.y; op.z = s.z; op.type = s.type;
+    op.re = th.re_n; op.tn = th.t_n; op.all_types = all_types;
+    const Real R2 = filt_r2(th.re_n);
+    for (int c0 = s0; c0 < s1; c0 += WAVE) {
+        const int i = c0 + (int)threadIdx.x;
+        const bool in = i < s1;
+        op.load_self(i, in);
+        Real lo[3], hi[3];
+        wave_box(op.active, op.xi, op.yi, op.zi, lo, hi);
+        if (!ballot(op.active)) continue;
+        stage(g, cs, cell, lo, hi, R2, buf, op);
+        if (op.active) {
+            s.n[i] = op.acc;
+            if (with_pressure) {
+                const double t = one_over_alpha * (op.acc - n0) / n0;
+                s.p[i] = (Real)(t > 0.0 ? t : 0.0);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// dummy pressure p_i = p_normal(i) (MPS_GPU_EXPL.cpp:228-236); no partner -> 0
+template <typename Real>
+__global__ void __launch_bounds__(256) k_dummy_p(int n, StateView<Real> s)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || s.type[i] != 2) return;
+    const int m = s.normal[i];
+    s.p[i] = m >= 0 ? s.p[m] : (Real)0;
+}
+
+// ------------------------------------------------------------ viscosity ---
+template <typename Real>
+struct OpVisc {
+    const Real *x, *y, *z, *u, *v, *w;
+    Real re, tl;
+    int i;
+    bool active;
+    Real xi, yi, zi, ui, vi, wi;
+    Real lx, ly, lz;
+
+    __device__ void load_cand(int j, Cand<Real>& c)
+    {
+        c.x = x[j]; c.y = y[j]; c.z = z[j]; c.j = j;
+        c.a = u[j]; c.b = v[j]; c.c = w[j];
+    }
+    __device__ void process(const Cand<Real>* buf, int cnt)
+    {
+        for (int k = 0; k < cnt; k++) {
+            const Cand<Real> c = buf[k];
+            const Real dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
+            const Real rr = rr3(dx, dy, dz);
+            const bool inside = rr < tl && c.j != i;
+            const Real wt = inside ? __builtin_fma(re, rsq(rr), (Real)-1) : (Real)0;
+            lx = __builtin_fma(wt, c.a - ui, lx);
+            ly = __builtin_fma(wt, c.b - vi, ly);
+            lz = __builtin_fma(wt, c.c - wi, lz);
+        }
+    }
+};
+
+template <typename Real>
+__global__ void __launch_bounds__(64) k_visc(Grid g, const int* __restrict__ cs, StateView<Real> a,
+                                             StateView<Real> b, Thresh<Real> th, StepConst kc,
+                                             const DevScalars* __restrict__ sc)
+{
+    __shared__ Cand<Real> buf[Cap<Real>::v];
+    const int cell = xcd_cell(g.ncells);
+    if (cell >= g.ncells) return;
+    const int s0 = cs[cell], s1 = cs[cell + 1];
+    const double dt = sc->dt, coef = sc->coef;
+    OpVisc<Real> op;
+    op.x = a.x; op.y = a.y; op.z = a.z; op.u = a.u; op.v = a.v; op.w = a.w;
+    op.re = th.re_lap; op.tl = th.t_lap;
+    const Real R2 = filt_r2(th.re_lap);
+    for (int c0 = s0; c0 < s1; c0 += WAVE) {
+        const int i = c0 + (int)threadIdx.x;
+        const bool in = i < s1;
+        op.i = i;
+        op.active = false;
+        op.lx = op.ly = op.lz = (Real)0;
+        if (in) {
+            op.xi = a.x[i]; op.yi = a.y[i]; op.zi = a.z[i];
+            op.ui = a.u[i]; op.vi = a.v[i]; op.wi = a.w[i];
+            op.active = a.type[i] == 0;
+        }
+        Real lo[3], hi[3];
+        wave_box(op.active, op.xi, op.yi, op.zi, lo, hi);
+        if (ballot(op.active)) stage(g, cs, cell, lo, hi, R2, buf, op);
+        if (in) {
+            Real xo = op.xi, yo = op.yi, zo = op.zi, uo = op.ui, vo = op.vi, wo = op.wi;
+            if (op.active) {
+                const Real dtr = (Real)dt, nc = (Real)(kc.nu * coef);
+                uo = op.ui + dtr * (nc * op.lx + (Real)kc.g[0]);
+                vo = op.vi + dtr * (nc * op.ly + (Real)kc.g[1]);
+                wo = op.wi + dtr * (nc * op.lz + (Real)kc.g[2]);
+                xo = op.xi + dtr * uo; yo = op.yi + dtr * vo; zo = op.zi + dtr * wo;
+            }
+            b.x[i] = xo; b.y[i] = yo; b.z[i] = zo; b.u[i] = uo; b.v[i] = vo; b.w[i] = wo;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ collision ---
+template <typename Real>
+struct OpCol {
+    const Real *x, *y, *z, *u, *v, *w;
+    Real tc, rate;
+    int i, hits;
+    bool active;
+    Real xi, yi, zi, ui, vi, wi;
+    Real cx, cy, cz;
+
+    __device__ void load_cand(int j, Cand<Real>& c)
+    {
+        c.x = x[j]; c.y = y[j]; c.z = z[j]; c.j = j;
+        c.a = u[j]; c.b = v[j]; c.c = w[j];
+    }
+    __device__ void process(const Cand<Real>* buf, int cnt)
+    {
+        for (int k = 0; k < cnt; k++) {
+            const Cand<Real> c = buf[k];
+            const Real dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
+            const Real rr = rr3(dx, dy, dz);
+            if (rr < tc && c.j != i) {
+                // v = 0.5 (u_j - u_i) . r / |r|; the sign decides, formed as the reference forms it
+                const Real hx = (Real)0.5 * (c.a - ui), hy = (Real)0.5 * (c.b - vi),
+                           hz = (Real)0.5 * (c.c - wi);
+                Real dot = hx * dx;
+                dot = dot + hy * dy;
+                dot = dot + hz * dz;
+                if (dot <= (Real)0) {
+                    const Real ods = rsq(rr);
+                    const Real f = rate * (dot * ods) * ods;
+                    cx = __builtin_fma(f, dx, cx);
+                    cy = __builtin_fma(f, dy, cy);
+                    cz = __builtin_fma(f, dz, cz);
+                    hits++;
+                }
+            }
+        }
+    }
+};
+
+template <typename Real>
+__global__ void __launch_bounds__(64) k_collision(Grid g, const int* __restrict__ cs, StateView<Real> a,
+                                                  StateView<Real> b, Thresh<Real> th, StepConst kc,
+                                                  DevScalars* __restrict__ sc)
+{
+    __shared__ Cand<Real> buf[Cap<Real>::v];
+    const int cell = xcd_cell(g.ncells);
+    if (cell >= g.ncells) return;
+    const int s0 = cs[cell], s1 = cs[cell + 1];
+    const double dt = sc->dt;
+    OpCol<Real> op;
+    op.x = a.x; op.y = a.y; op.z = a.z; op.u = a.u; op.v = a.v; op.w = a.w;
+    op.tc = th.t_col; op.rate = (Real)kc.col_rate;
+    const Real R2 = filt_r2(th.col);
+    int hits = 0;
+    for (int c0 = s0; c0 < s1; c0 += WAVE) {
+        const int i = c0 + (int)threadIdx.x;
+        const bool in = i < s1;
+        op.i = i;
+        op.active = false;
+        op.hits = 0;
+        op.cx = op.cy = op.cz = (Real)0;
+        if (in) {
+            op.xi = a.x[i]; op.yi = a.y[i]; op.zi = a.z[i];
+            op.ui = a.u[i]; op.vi = a.v[i]; op.wi = a.w[i];
+            op.active = a.type[i] == 0;
+        }
+        Real lo[3], hi[3];
+        wave_box(op.active, op.xi, op.yi, op.zi, lo, hi);
+        if (ballot(op.active)) stage(g, cs, cell, lo, hi, R2, buf, op);
+        if (in) {
+            Real cx = op.active ? op.cx : (Real)0, cy = op.active ? op.cy : (Real)0,
+                 cz = op.active ? op.cz : (Real)0;
+            const Real dtr = (Real)dt;
+            b.u[i] = op.ui + cx; b.v[i] = op.vi + cy; b.w[i] = op.wi + cz;
+            b.x[i] = op.xi + dtr * cx; b.y[i] = op.yi + dtr * cy; b.z[i] = op.zi + dtr * cz;
+            if (op.active) hits += op.hits;
+        }
+        __syncthreads();
+    }
+    hits = wave_sum(hits);
+    if (threadIdx.x == 0 && hits) atomicAdd(&sc->ncol, hits);
+}
+
+// ------------------------------------------------------ pressure gradient ---
+template <typename Real>
+struct OpGrad {
+    const Real *x, *y, *z, *p;
+    const int* type;
+    Real re, tn, re2;
+    int i, pass, skip_dummy;
+    bool active;
+    Real xi, yi, zi, ph;
+    Real gx, gy, gz;
+
+    __device__ void load_cand(int j, Cand<Real>& c)
+    {
+        c.x = x[j]; c.y = y[j]; c.z = z[j]; c.j = j;
+        c.t = type[j]; c.a = p[j];
+    }
+    __device__ void process(const Cand<Real>* buf, int cnt)
+    {
+        if (pass == 0) {
+            // p_hat = min(p_i, p_j : type_j != 2, |r|^2 <= R0^2)
+            for (int k = 0; k < cnt; k++) {
+                const Real cxk = buf[k].x, cyk = buf[k].y, czk = buf[k].z, pk = buf[k].a;
+                const int tk = buf[k].t;
+                const Real rr = rr3(cxk - xi, cyk - yi, czk - zi);
+                const bool near = tk != 2 && rr <= re2 && pk < ph;
+                ph = near ? pk : ph;
+            }
+            return;
+        }
+        for (int k = 0; k < cnt; k++) {
+            const Cand<Real> c = buf[k];
+            const Real dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
+            const Real rr = rr3(dx, dy, dz);
+            const bool inside = rr < tn && c.j != i && !(skip_dummy && c.t == 2);
+            const Real ri = rsq(rr);
+            const Real f = inside ? (c.a - ph) * (ri * ri) * __builtin_fma(re, ri, (Real)-1) : (Real)0;
+            gx = __builtin_fma(f, dx, gx);
+            gy = __builtin_fma(f, dy, gy);
+            gz = __builtin_fma(f, dz, gz);
+        }
+    }
+};
+
+template <typename Real>
+__global__ void __launch_bounds__(64) k_gradp(Grid g, const int* __restrict__ cs, StateView<Real> a,
+                                              StateView<Real> b, Thresh<Real> th, StepConst kc,
+                                              DevScalars* __restrict__ sc)
+{
+    __shared__ Cand<Real> buf[Cap<Real>::v];
+    const int cell = xcd_cell(g.ncells);
+    if (cell >= g.ncells) return;
+    const int s0 = cs[cell], s1 = cs[cell + 1];
+    const double dt = sc->dt, n0 = sc->n0;
+    OpGrad<Real> op;
+    op.x = a.x; op.y = a.y; op.z = a.z; op.p = a.p; op.type = a.type;
+    op.re = th.re_n; op.tn = th.t_n; op.re2 = th.re_n2; op.skip_dummy = kc.gradp_skip_dummy;
+    const Real R2 = filt_r2(th.re_n);
+    float umax2 = 0.0f;
+    for (int c0 = s0; c0 < s1; c0 += WAVE) {
+        const int i = c0 + (int)threadIdx.x;
+        const bool in = i < s1;
+        op.i = i;
+        op.active = false;
+        op.pass = 0;
+        op.gx = op.gy = op.gz = (Real)0;
+        Real ui = 0, vi = 0, wi = 0;
+        if (in) {
+            op.xi = a.x[i]; op.yi = a.y[i]; op.zi = a.z[i];
+            ui = a.u[i]; vi = a.v[i]; wi = a.w[i];
+            op.ph = a.p[i];
+            op.active = a.type[i] == 0;
+        }
+        Real lo[3], hi[3];
+        wave_box(op.active, op.xi, op.yi, op.zi, lo, hi);
+        if (ballot(op.active)) {
+            StageResult r = stage(g, cs, cell, lo, hi, R2, buf, op);
+            op.pass = 1;
+            if (!r.overflowed) {
+                op.process(buf, r.count);
+            } else {
+                __syncthreads();
+                stage(g, cs, cell, lo, hi, R2, buf, op);
+            }
+        }
+        if (in) {
+            Real xo = op.xi, yo = op.yi, zo = op.zi, uo = ui, vo = vi, wo = wi;
+            if (op.active) {
+                const Real s = (Real)(-dt / kc.rho * (double)kc.dim / n0);
+                const Real du = s * op.gx, dv = s * op.gy, dw = s * op.gz;
+                const Real dtr = (Real)dt;
+                uo = ui + du; vo = vi + dv; wo = wi + dw;
+                xo = op.xi + dtr * du; yo = op.yi + dtr * dv; zo = op.zi + dtr * dw;
+                const float m2 = (float)(uo * uo + vo * vo + wo * wo);
+                umax2 = m2 > umax2 ? m2 : umax2;
+            }
+            b.x[i] = xo; b.y[i] = yo; b.z[i] = zo; b.u[i] = uo; b.v[i] = vo; b.w[i] = wo;
+        }
+        __syncthreads();
+    }
+    umax2 = wave_max(umax2);
+    if (threadIdx.x == 0 && umax2 > 0.0f) atomicMax(&sc->maxu2_bits, __float_as_uint(umax2));
+}
+
+// ------------------------------------------------------------------- dt ---
+// MPS_GPU::update_dt (MPS_GPU.cpp:761-777): dt = min(dt_max, CFL dp / max|u|);
+// the flag replaces the reference's exit(1).  time accumulates dt.
+__global__ void k_dt(StepConst kc, DevScalars* sc)
+{
+    if (threadIdx.x != 0) return;
+    double m2 = (double)__uint_as_float(sc->maxu2_bits);
+    if (kc.umax2_static > m2) m2 = kc.umax2_static;
+    double dt = kc.dt_max;
+    if (m2 > 0.0) {
+        const double t = kc.cfl * kc.dp / sqrt(m2);
+        if (t < dt) dt = t;
+    }
+    if (dt < kc.dt_min) sc->err = 1;
+    sc->dt = dt;
+    sc->time += dt;
+    sc->maxu2_bits = 0u;
+    sc->ncol = 0;
+}
+
+// -------------------------------------------------------------- calOnce ---
+// MPS::calOnce (MPS_CAL_expl_v1.0/MPS.cpp:311-451)
+__global__ void __launch_bounds__(256) k_nmax(int np, const double* __restrict__ n, DevScalars* sc)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < np) atomicMax(&sc->nmax_bits, (unsigned long long)__double_as_longlong(n[i]));
+}
+__global__ void __launch_bounds__(256) k_nargmax(int np, const double* __restrict__ n, DevScalars* sc)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < np && (unsigned long long)__double_as_longlong(n[i]) == sc->nmax_bits) atomicMin(&sc->argmax, i);
+}
+
+// lambda = sum_{j != i*} |r|^2 w(|r|; R0) / n0 around the first maximum, then
+// the Laplacian coefficient 2 d / (n0 lambda).
+template <typename Real>
+__global__ void __launch_bounds__(64) k_lambda(Grid g, const int* __restrict__ cs, StateView<Real> s,
+                                               Thresh<Real> th, int dim, DevScalars* sc)
+{
+    const int am = sc->argmax;
+    const double n0 = __longlong_as_double((long long)sc->nmax_bits);
+    int cell = 0;
+    {   // cell of the argmax particle: last c with cs[c] <= am
+        int lo = 0, hi = g.ncells - 1;
+        while (lo < hi) { int mid = (lo + hi + 1) >> 1; if (cs[mid] <= am) lo = mid; else hi = mid - 1; }
+        cell = lo;
+    }
+    int cx, cy, cz;
+    cell_xyz(g, cell, cx, cy, cz);
+    const Real xi = s.x[am], yi = s.y[am], zi = s.z[am];
+    double acc = 0.0;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const int xx = cx + dx, yy = cy + dy, zz = cz + dz;
+                if (xx < 0 || yy < 0 || zz < 0 || xx >= g.dx || yy >= g.dy || zz >= g.dz) continue;
+                const int c = zz * g.sheet + yy * g.dx + xx;
+                for (int j = cs[c] + (int)threadIdx.x; j < cs[c + 1]; j += WAVE) {
+                    if (j == am) continue;
+                    const Real ex = s.x[j] - xi, ey = s.y[j] - yi, ez = s.z[j] - zi;
+                    if (!(rr3(ex, ey, ez) < th.t_n)) continue;
+                    const double r2 = (double)ex * ex + (double)ey * ey + (double)ez * ez;
+                    const double r = sqrt(r2);
+                    acc += r2 * ((double)th.re_n / r - 1.0);
+                }
+            }
+    acc = wave_sum(acc);
+    if (threadIdx.x == 0) {
+        sc->n0 = n0;
+        sc->lambda = acc / n0;
+        sc->coef = 2.0 * dim / (n0 * sc->lambda);
+    }
+}
+
+// i_normal: nearest type-1 particle with |r|^2 < 2 R0^2, first strict minimum
+// in index order (MPS_CAL_expl_v1.0/MPS.cpp:423-444); -1 when there is none.
+template <typename Real>
+__global__ void __launch_bounds__(256) k_normals(Grid g, const int* __restrict__ cs, int np,
+                                                 StateView<Real> s, Thresh<Real> th)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    if (s.type[i] != 2) { s.normal[i] = -1; return; }
+    int lo = 0, hi = g.ncells - 1;
+    while (lo < hi) { int mid = (lo + hi + 1) >> 1; if (cs[mid] <= i) lo = mid; else hi = mid - 1; }
+    int cx, cy, cz;
+    cell_xyz(g, lo, cx, cy, cz);
+    const Real xi = s.x[i], yi = s.y[i], zi = s.z[i];
+    Real best = ((Real)2 * th.re_n) * th.re_n;
+    int bj = -1;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const int xx = cx + dx, yy = cy + dy, zz = cz + dz;
+                if (xx < 0 || yy < 0 || zz < 0 || xx >= g.dx || yy >= g.dy || zz >= g.dz) continue;
+                const int c = zz * g.sheet + yy * g.dx + xx;
+                for (int j = cs[c]; j < cs[c + 1]; j++) {
+                    if (s.type[j] != 1) continue;
+                    const Real rr = rr3(s.x[j] - xi, s.y[j] - yi, s.z[j] - zi);
+                    if (rr < best) { best = rr; bj = j; }
+                }
+            }
+    s.normal[i] = bj;
+}
+
+// ------------------------------------------------------- neighbour lists ---
+// Within-cutoff neighbour lists in the reference visiting order (ascending j),
+// which = 0: R0, 1: RLAP.  fill = 0 counts into row_ptr[i + 1].
+template <typename Real>
+__global__ void __launch_bounds__(256) k_nlist(Grid g, const int* __restrict__ cs, int np,
+                                               StateView<Real> s, Real t, int* __restrict__ row_ptr,
+                                               int* __restrict__ col, int fill)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    int lo = 0, hi = g.ncells - 1;
+    while (lo < hi) { int mid = (lo + hi + 1) >> 1; if (cs[mid] <= i) lo = mid; else hi = mid - 1; }
+    int cx, cy, cz;
+    cell_xyz(g, lo, cx, cy, cz);
+    const Real xi = s.x[i], yi = s.y[i], zi = s.z[i];
+    int k = fill ? row_ptr[i] : 0;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const int xx = cx + dx, yy = cy + dy, zz = cz + dz;
+                if (xx < 0 || yy < 0 || zz < 0 || xx >= g.dx || yy >= g.dy || zz >= g.dz) continue;
+                const int c = zz * g.sheet + yy * g.dx + xx;
+                for (int j = cs[c]; j < cs[c + 1]; j++) {
+                    if (j == i) continue;
+                    if (rr3(s.x[j] - xi, s.y[j] - yi, s.z[j] - zi) < t) {
+                        if (fill) col[k] = j;
+                        k++;
+                    }
+                }
+            }
+    if (!fill) row_ptr[i + 1] = k;
+}
+
+// ------------------------------------------------------------- launchers ---
+static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+template <typename Real>
+void launch_pnd(const Grid& g, const int* cs, StateView<Real> s, const Thresh<Real>& th,
+                const StepConst& k, DevScalars* sc, int all_types, int with_pressure, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_pnd<Real>, dim3(xcd_grid(g.ncells)), dim3(64), 0, st, g, cs, s, th, sc,
+                       k.rho * k.cs * k.cs, all_types, with_pressure);
+}
+
+template <typename Real>
+void launch_dummy_p(int n, StateView<Real> s, hipStream_t st)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_dummy_p<Real>, dim3(cdiv(n, 256)), dim3(256), 0, st, n, s);
+}
+
+template <typename Real>
+void launch_visc(const Grid& g, const int* cs, StateView<Real> a, StateView<Real> b,
+                 const Thresh<Real>& th, const StepConst& k, DevScalars* sc, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_visc<Real>, dim3(xcd_grid(g.ncells)), dim3(64), 0, st, g, cs, a, b, th, k, sc);
+}
+
+template <typename Real>
+void launch_collision(const Grid& g, const int* cs, StateView<Real> a, StateView<Real> b,
+                      const Thresh<Real>& th, const StepConst& k, DevScalars* sc, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_collision<Real>, dim3(xcd_grid(g.ncells)), dim3(64), 0, st, g, cs, a, b, th, k, sc);
+}
+
+template <typename Real>
+void launch_gradp(const Grid& g, const int* cs, StateView<Real> a, StateView<Real> b,
+                  const Thresh<Real>& th, const StepConst& k, DevScalars* sc, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_gradp<Real>, dim3(xcd_grid(g.ncells)), dim3(64), 0, st, g, cs, a, b, th, k, sc);
+}
+
+void launch_dt(const StepConst& k, DevScalars* sc, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_dt, dim3(1), dim3(64), 0, st, k, sc);
+}
+
+// n over every particle, n0 = first maximum, lambda and coef, normals.
+// The caller zeroes nmax_bits and sets argmax to INT_MAX beforehand.
+template <typename Real>
+void launch_calonce(const Grid& g, const int* cs, int np, StateView<Real> s, const Thresh<Real>& th,
+                    const StepConst& k, DevScalars* sc, hipStream_t st)
+{
+    launch_pnd<Real>(g, cs, s, th, k, sc, 1, 0, st);
+    if (np > 0) {
+        hipLaunchKernelGGL(k_nmax, dim3(cdiv(np, 256)), dim3(256), 0, st, np, s.n, sc);
+        hipLaunchKernelGGL(k_nargmax, dim3(cdiv(np, 256)), dim3(256), 0, st, np, s.n, sc);
+        hipLaunchKernelGGL(k_lambda<Real>, dim3(1), dim3(64), 0, st, g, cs, s, th, k.dim, sc);
+        hipLaunchKernelGGL(k_normals<Real>, dim3(cdiv(np, 256)), dim3(256), 0, st, g, cs, np, s, th);
+    }
+}
+
+template <typename Real>
+void launch_nlist(const Grid& g, const int* cs, int np, StateView<Real> s, const Thresh<Real>& th,
+                  int which, int* row_ptr, int* col, int fill, hipStream_t st)
+{
+    if (np == 0) return;
+    hipLaunchKernelGGL(k_nlist<Real>, dim3(cdiv(np, 256)), dim3(256), 0, st, g, cs, np, s,
+                       which ? th.t_lap : th.t_n, row_ptr, col, fill);
+}
+
+#define MPS_INST(R)                                                                                   \
+    template void launch_pnd<R>(const Grid&, const int*, StateView<R>, const Thresh<R>&,            \
+                                const StepConst&, DevScalars*, int, int, hipStream_t);               \
+    template void launch_dummy_p<R>(int, StateView<R>, hipStream_t);                                 \
+    template void launch_visc<R>(const Grid&, const int*, StateView<R>, StateView<R>,               \
+                                 const Thresh<R>&, const StepConst&, DevScalars*, hipStream_t);      \
+    template void launch_collision<R>(const Grid&, const int*, StateView<R>, StateView<R>,          \
+                                      const Thresh<R>&, const StepConst&, DevScalars*, hipStream_t); \
+    template void launch_gradp<R>(const Grid&, const int*, StateView<R>, StateView<R>,              \
+                                  const Thresh<R>&, const StepConst&, DevScalars*, hipStream_t);     \
+    template void launch_calonce<R>(const Grid&, const int*, int, StateView<R>, const Thresh<R>&,   \
+                                    const StepConst&, DevScalars*, hipStream_t);                     \
+    template void launch_nlist<R>(const Grid&, const int*, int, StateView<R>, const Thresh<R>&, int, \
+                                  int*, int*, int, hipStream_t);
+MPS_INST(float)
+MPS_INST(double)
+#undef MPS_INST
+
+}  // namespace mps
