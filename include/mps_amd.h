@@ -135,6 +135,37 @@ int mps_neighbor_list(mps_ctx* c, int which, int* row_ptr, int* col, long cap, l
 int mps_stage_times(mps_ctx* c, double* ms8, int reset);
 int mps_enable_timing(mps_ctx* c, int on);
 
+/* ------------------------------------------------ pressure Poisson (semi-implicit) */
+
+/* MPS_GPU::cal_n alone (MPS_CAL_v4.0/MPS_CPU.cpp:573-614): n* of the current state. */
+int mps_stage_pnd(mps_ctx* c);
+/* MPS_CPU::buildPoisson (MPS_CAL_v4.0/MPS_CPU.cpp:619-710) as CSR on the device,
+ * from the current n, positions and dt.  alpha is the compressibility of the
+ * diagonal term (1/(rho c^2) in the reference; 0 gives the textbook form). */
+int mps_ppe_assemble(mps_ctx* c, double alpha, long* nnz);
+int mps_ppe_info(mps_ctx* c, int* nrows, long* nnz);
+/* Copy the assembled system: row_ptr (nrows + 1), col / val (nnz, FP64 values),
+ * b, current iterate x, Dirichlet flags bc (nrows).  Any pointer may be NULL. */
+int mps_ppe_get(mps_ctx* c, int* row_ptr, int* col, double* val, double* b, double* x,
+                unsigned char* bc);
+int mps_ppe_get_x(mps_ctx* c, double* x);
+/* MPS_CPU::DoCG (MPS_CAL_v4.0/MPS_CPU.cpp:829-888) / cudaker::CG
+ * (MPS_CAL_v4.0/cuda_ker.cu:171-297): solve A x = b to |b - A x| <= tol |b|
+ * (FP64 residual).  mixed 1: FP32-SpMV CG inside FP64 iterative refinement;
+ * 0: FP64 CG.  x0_kind 0: zero, 1: current pressure (previous step),
+ * 2: host array x0, 3: the learned guess left by mps_mlp_infer.
+ * Returns MPS_ERR_STATE when maxit inner iterations did not reach tol. */
+int mps_ppe_solve(mps_ctx* c, int mixed, int x0_kind, const double* x0, double tol, int maxit,
+                  int* iters, double* relres);
+/* MPS_CPU::pressCorr (MPS_CAL_v4.0/MPS_CPU.cpp:734-743): p = max(x, 0) into the state. */
+int mps_ppe_press_corr(mps_ctx* c);
+/* MPS_CPU_EXPL::step (MPS_CAL_v4.0/MPS_CPU_EXPL.cpp:79-135): dt, viscosity,
+ * collision, n*, Poisson build + mixed-precision CG to tol, pressCorr,
+ * pressure-gradient correction.  Reports total CG iterations and device
+ * milliseconds spent in the solves. */
+int mps_step_semi(mps_ctx* c, int nsteps, double alpha, int x0_kind, double tol, int* iters_total,
+                  double* solve_ms);
+
 #ifdef __cplusplus
 }
 #endif

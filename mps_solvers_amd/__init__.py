@@ -87,6 +87,13 @@ def lib():
         L.mps_load_case.argtypes = [C.c_char_p, C.POINTER(Params)]
         L.mps_write_case.argtypes = [C.c_char_p, C.POINTER(Params)]
         L.mps_params_default.argtypes = [C.POINTER(Params)]
+        L.mps_stage_pnd.argtypes = [vp]
+        L.mps_ppe_assemble.argtypes = [vp, C.c_double, C.POINTER(C.c_long)]
+        L.mps_ppe_info.argtypes = [vp, I, C.POINTER(C.c_long)]
+        L.mps_ppe_get.argtypes = [vp, I, I, D, D, D, C.POINTER(C.c_ubyte)]
+        L.mps_ppe_solve.argtypes = [vp, C.c_int, C.c_int, D, C.c_double, C.c_int, I, D]
+        L.mps_ppe_press_corr.argtypes = [vp]
+        L.mps_step_semi.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.c_double, I, D]
         _lib = L
     return _lib
 
@@ -272,6 +279,48 @@ class Solver:
         _check(lib().mps_neighbor_list(self.h, which, _ip(rp), _ip(col), nnz.value, C.byref(nnz)),
                "mps_neighbor_list")
         return rp, col[: nnz.value]
+
+    # ------------------------------------------------ pressure Poisson
+    def stage_pnd(self):
+        _check(lib().mps_stage_pnd(self.h), "mps_stage_pnd")
+
+    def ppe_assemble(self, alpha):
+        nnz = C.c_long()
+        _check(lib().mps_ppe_assemble(self.h, float(alpha), C.byref(nnz)), "mps_ppe_assemble")
+        return nnz.value
+
+    def ppe_get(self):
+        n, nnz = C.c_int(), C.c_long()
+        _check(lib().mps_ppe_info(self.h, C.byref(n), C.byref(nnz)), "mps_ppe_info")
+        rp = np.zeros(n.value + 1, np.int32)
+        col = np.zeros(max(nnz.value, 1), np.int32)
+        val = np.zeros(max(nnz.value, 1))
+        b = np.zeros(n.value)
+        x = np.zeros(n.value)
+        bc = np.zeros(n.value, np.uint8)
+        _check(lib().mps_ppe_get(self.h, _ip(rp), _ip(col), _dp(val), _dp(b), _dp(x),
+                                 bc.ctypes.data_as(C.POINTER(C.c_ubyte))), "mps_ppe_get")
+        return dict(row_ptr=rp, col=col[: nnz.value], val=val[: nnz.value], b=b, x=x, bc=bc)
+
+    def ppe_solve(self, mixed=True, x0_kind=0, x0=None, tol=1e-8, maxit=100000, check=True):
+        """Returns (iterations, relative residual |b - A x| / |b| in FP64)."""
+        it, rel = C.c_int(), C.c_double()
+        x0 = None if x0 is None else np.ascontiguousarray(x0, np.float64)
+        rc = lib().mps_ppe_solve(self.h, int(mixed), int(x0_kind), _dp(x0), float(tol), int(maxit),
+                                 C.byref(it), C.byref(rel))
+        if check:
+            _check(rc, "mps_ppe_solve")
+        return it.value, rel.value
+
+    def ppe_press_corr(self):
+        _check(lib().mps_ppe_press_corr(self.h), "mps_ppe_press_corr")
+
+    def step_semi(self, nsteps=1, alpha=0.0, x0_kind=1, tol=1e-8):
+        """Semi-implicit steps; returns (total CG iterations, solve milliseconds)."""
+        it, ms = C.c_int(), C.c_double()
+        _check(lib().mps_step_semi(self.h, nsteps, float(alpha), int(x0_kind), float(tol),
+                                   C.byref(it), C.byref(ms)), "mps_step_semi")
+        return it.value, ms.value
 
     def enable_timing(self, on=True):
         _check(lib().mps_enable_timing(self.h, int(on)), "mps_enable_timing")

@@ -17,6 +17,7 @@
 
 #include "../../include/mps_amd.h"
 #include "mps_kernels.hpp"
+#include "mps_ctx.hpp"
 
 using namespace mps;
 
@@ -210,13 +211,6 @@ extern "C" int mps_scene_dambreak(int dim, int fx, int fy, int fz, int nx, int n
 
 namespace {
 
-enum { T_DT, T_SORT, T_VISC, T_COL, T_PND, T_DUMMY, T_GRAD, T_STEP, T_N };
-
-struct Thr {
-    Thresh<float> f;
-    Thresh<double> d;
-};
-
 float smallest_f(float a, bool strict)
 {
     // smallest float t with sqrtf(t) >= a (strict: > a)
@@ -236,42 +230,6 @@ double smallest_d(double a, bool strict)
 }
 
 }  // namespace
-
-struct mps_ctx {
-    mps_params P;
-    int prec = 0;
-    int nt = 0, nb2 = 0, nb1 = 0;
-    Grid g{};
-    Thr th{};
-    StepConst kc{};
-    hipStream_t st = nullptr;
-    int key_bits = 0;
-    // two copies of the state; kin selects x..w, aux selects p, n, type, id, normal
-    void* f[2][7] = {};
-    double* n[2] = {};
-    int *type[2] = {}, *id[2] = {}, *normal[2] = {};
-    int kin = 0, aux = 0;
-    uint32_t *k0 = nullptr, *v0 = nullptr, *k1 = nullptr, *v1 = nullptr, *hist = nullptr;
-    uint32_t* perm = nullptr;   // last sort's permutation (one of v0/v1)
-    int *inv = nullptr, *ntmp = nullptr, *cs = nullptr;
-    DevScalars* sc = nullptr;
-    long steps = 0;
-    bool timing = false;
-    hipEvent_t ev[T_N][2] = {};
-    double ms[T_N] = {};
-    std::vector<void*> allocs;
-
-    template <typename Real>
-    StateView<Real> view(int k, int a)
-    {
-        StateView<Real> s;
-        s.x = (Real*)f[k][0]; s.y = (Real*)f[k][1]; s.z = (Real*)f[k][2];
-        s.u = (Real*)f[k][3]; s.v = (Real*)f[k][4]; s.w = (Real*)f[k][5];
-        s.p = (Real*)f[a][6]; s.n = n[a]; s.type = type[a]; s.id = id[a]; s.normal = normal[a];
-        return s;
-    }
-    size_t rsize() const { return prec ? sizeof(double) : sizeof(float); }
-};
 
 static int dmalloc(mps_ctx* c, void** p, size_t bytes)
 {
@@ -370,7 +328,7 @@ static const Thresh<Real>& thr(mps_ctx* c);
 template <> const Thresh<float>& thr<float>(mps_ctx* c) { return c->th.f; }
 template <> const Thresh<double>& thr<double>(mps_ctx* c) { return c->th.d; }
 
-enum Stage { S_VISC, S_COL, S_PND, S_DUMMY, S_GRAD };
+enum Stage { S_VISC, S_COL, S_PND, S_DUMMY, S_GRAD, S_PND_ONLY };
 
 template <typename Real>
 static int run_stage_t(mps_ctx* c, Stage s)
@@ -401,6 +359,11 @@ static int run_stage_t(mps_ctx* c, Stage s)
         launch_dummy_p<Real>(c->nt, cur, c->st);
         toc(c, T_DUMMY);
         break;
+    case S_PND_ONLY:
+        tic(c, T_PND);
+        launch_pnd<Real>(c->g, c->cs, cur, thr<Real>(c), c->kc, c->sc, 0, 0, c->st);
+        toc(c, T_PND);
+        break;
     case S_GRAD:
         tic(c, T_GRAD);
         launch_gradp<Real>(c->g, c->cs, cur, alt, thr<Real>(c), c->kc, c->sc, c->st);
@@ -420,6 +383,9 @@ static int do_sort(mps_ctx* c)
     toc(c, T_SORT);
     return rc;
 }
+
+int mps_ctx_divide(mps_ctx* c) { return do_sort(c); }
+int mps_ctx_stage(mps_ctx* c, int stage) { return run_stage(c, (Stage)stage); }
 
 // One step.  rebin_every_move = 0: MPS_CAL_/MPS_GPU.cpp:611-714 (re-bin, visc,
 // collision, n, p, grad p); 1: MPS_CAL_expl_v1.0/MPS_GPU_EXPL.cpp:85-147
@@ -587,6 +553,8 @@ extern "C" int mps_destroy(mps_ctx* c)
 {
     if (!c) return MPS_OK;
     if (c->st) hipStreamSynchronize(c->st);
+    mps_ppe_free(c);
+    mps_mlp_free(c);
     for (void* p : c->allocs) hipFree(p);
     for (int t = 0; t < T_N; t++)
         for (int e = 0; e < 2; e++)

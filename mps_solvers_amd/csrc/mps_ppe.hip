@@ -1,0 +1,367 @@
+// mps_ppe.hip -- pressure Poisson equation of the semi-implicit MPS step:
+// CSR assembly and a hand-written conjugate-gradient solver on gfx950.
+//
+// Assembly restates MPS_CPU::buildPoisson (MPS_CAL_v4.0/MPS_CPU.cpp:619-710):
+//   bc_i = n_i < beta n0; rows with bc_i or type 2 are identity with b = 0;
+//   otherwise A_ij = w(|r_ij|; RLAP) for j != i, type_j != 2, j not bc, and
+//   A_ii = -sum_{j != i, type_j != 2} w_ij - rho lambda alpha n0 / (2 d dt^2),
+//   b_i = lambda rho / (2 d dt^2) (n0 - n_i).
+// Columns ascend (the reference's visiting order).  Values are formed in FP64
+// and stored twice: FP32 for the inner CG SpMV, FP64 for the outer residual.
+//
+// The solver replaces MPS_CPU::DoCG (MPS_CPU.cpp:829-888) and its cuBLAS twin
+// (MPS_CAL_v4.0/cuda_ker.cu:171-297): CG on the CSR matrix with FP32 SpMV
+// inside an FP64 iterative-refinement loop (or plain FP64 CG).  Reductions are
+// deterministic: per-block FP64 partials, summed in a fixed order by the last
+// block to finish.  No kernel waits on another block.
+#include "mps_device.hpp"
+#include "mps_kernels.hpp"
+#include "mps_ppe.hpp"
+#include "mps_stage.hpp"
+
+namespace mps {
+
+// ------------------------------------------------------------ row setup ---
+template <typename Real>
+__global__ void __launch_bounds__(256) k_ppe_rows(int n, StateView<Real> s, const DevScalars* __restrict__ sc,
+                                                  double beta, double btmp_over, int x0_kind,
+                                                  const double* __restrict__ x0_ext,
+                                                  unsigned char* __restrict__ bc, double* __restrict__ b,
+                                                  double* __restrict__ x, double* __restrict__ partial,
+                                                  PpeScalars* ps)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double n0 = sc->n0, lambda = sc->lambda, dt = sc->dt;
+    double bb = 0.0;
+    if (i < n) {
+        const double ni = s.n[i];
+        const bool isbc = ni < beta * n0;
+        const bool id = isbc || s.type[i] == 2;
+        // lambda rho / (2 d dt^2) = lambda * btmp_over / dt^2
+        const double bi = id ? 0.0 : lambda * btmp_over / (dt * dt) * (n0 - ni);
+        bc[i] = isbc ? 1 : 0;
+        b[i] = bi;
+        double xi = 0.0;
+        if (x0_kind == 1) xi = (double)s.p[i];
+        else if (x0_kind == 2) xi = x0_ext[i];
+        x[i] = isbc ? 0.0 : xi;
+        bb = bi * bi;
+    }
+    block_sum_last(bb, partial, &ps->ticket[0], &ps->bb);
+}
+
+// ------------------------------------------------------------- assembly ---
+template <typename Real>
+struct OpPpe {
+    const Real *x, *y, *z;
+    const int* type;
+    const unsigned char* bcv;
+    Real tl;
+    double re;
+    int i, fill;
+    bool active;
+    Real xi, yi, zi;
+    int cnt;
+    double diag;
+    long kpos, kdiag;
+    int* col;
+    float* v32;
+    double* v64;
+
+    __device__ void load_cand(int j, Cand<Real>& c)
+    {
+        c.x = x[j]; c.y = y[j]; c.z = z[j]; c.j = j;
+        c.t = type[j] | (bcv[j] ? 16 : 0);
+    }
+    __device__ void process(const Cand<Real>* buf, int nc)
+    {
+        for (int k = 0; k < nc; k++) {
+            const Real cx = buf[k].x, cy = buf[k].y, cz = buf[k].z;
+            const int cj = buf[k].j, ct = buf[k].t;
+            if (!active) continue;
+            if (cj == i) {
+                if (fill) { kdiag = kpos; col[kpos] = i; }
+                kpos++;
+                continue;
+            }
+            if ((ct & 15) == 2) continue;
+            const Real rr = rr3(cx - xi, cy - yi, cz - zi);
+            if (!(rr < tl)) continue;
+            const double dx = (double)cx - (double)xi, dy = (double)cy - (double)yi,
+                         dz = (double)cz - (double)zi;
+            const double w = __builtin_fma(re, rsq(dx * dx + dy * dy + dz * dz), -1.0);
+            diag -= w;
+            if (ct & 16) continue;   // Dirichlet neighbour: A_ij = 0
+            if (fill) { col[kpos] = cj; v64[kpos] = w; v32[kpos] = (float)w; }
+            kpos++;
+        }
+    }
+};
+
+template <typename Real>
+__global__ void __launch_bounds__(64) k_ppe_assemble(Grid g, const int* __restrict__ cs, StateView<Real> s,
+                                                     Thresh<Real> th, const unsigned char* __restrict__ bc,
+                                                     const DevScalars* __restrict__ sc, double aii_over,
+                                                     int fill, int* __restrict__ row_ptr,
+                                                     int* __restrict__ col, float* __restrict__ v32,
+                                                     double* __restrict__ v64)
+{
+    __shared__ Cand<Real> buf[Cap<Real>::v];
+    const int cell = xcd_cell(g.ncells);
+    if (cell >= g.ncells) return;
+    const int s0 = cs[cell], s1 = cs[cell + 1];
+    // rho lambda alpha n0 / (2 d dt^2)
+    const double dt = sc->dt;
+    const double aii = aii_over * sc->lambda * sc->n0 / (dt * dt);
+    OpPpe<Real> op;
+    op.x = s.x; op.y = s.y; op.z = s.z; op.type = s.type; op.bcv = bc;
+    op.tl = th.t_lap; op.re = (double)th.re_lap; op.fill = fill;
+    op.col = col; op.v32 = v32; op.v64 = v64;
+    const Real R2 = filt_r2(th.re_lap);
+    for (int c0 = s0; c0 < s1; c0 += WAVE) {
+        const int i = c0 + (int)threadIdx.x;
+        const bool in = i < s1;
+        op.i = i;
+        op.active = false;
+        op.diag = 0.0;
+        op.kpos = fill && in ? row_ptr[i] : 0;
+        op.kdiag = -1;
+        if (in) {
+            op.xi = s.x[i]; op.yi = s.y[i]; op.zi = s.z[i];
+            op.active = !bc[i] && s.type[i] != 2;
+        }
+        Real lo[3], hi[3];
+        wave_box(op.active, op.xi, op.yi, op.zi, lo, hi);
+        if (ballot(op.active)) stage(g, cs, cell, lo, hi, R2, buf, op);
+        if (in) {
+            if (!op.active) {
+                if (fill) {
+                    const long k = row_ptr[i];
+                    col[k] = i; v64[k] = 1.0; v32[k] = 1.0f;
+                } else {
+                    row_ptr[i] = 1;
+                }
+            } else if (fill) {
+                const double d = op.diag - aii;
+                v64[op.kdiag] = d;
+                v32[op.kdiag] = (float)d;
+            } else {
+                row_ptr[i] = (int)op.kpos;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ----------------------------------------------------------------- SpMV ---
+// CSR-vector: G lanes per row, lane-strided partial sums, then a shuffle tree
+// over the G lanes (fixed order: deterministic).
+template <int G, typename MV, typename V, typename Acc>
+__device__ __forceinline__ Acc row_dot(int row, int g, const int* __restrict__ rp,
+                                       const int* __restrict__ col, const MV* __restrict__ val,
+                                       const V* __restrict__ p)
+{
+    Acc a = (Acc)0;
+    const int k1 = rp[row + 1];
+    for (int k = rp[row] + g; k < k1; k += G) a += (Acc)val[k] * (Acc)p[col[k]];
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1) a += __shfl_xor(a, m);
+    return a;
+}
+
+// q = A p, pq = p . q
+template <int G, typename MV, typename V>
+__global__ void __launch_bounds__(256) k_spmv_dot(int n, const int* __restrict__ rp, const int* __restrict__ col,
+                                                  const MV* __restrict__ val, const V* __restrict__ p,
+                                                  V* __restrict__ q, double* __restrict__ partial,
+                                                  PpeScalars* ps)
+{
+    if (ps->done) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = t / G, g = t % G;
+    double d = 0.0;
+    if (row < n) {
+        const V a = row_dot<G, MV, V, V>(row, g, rp, col, val, p);
+        if (g == 0) {
+            q[row] = a;
+            d = (double)p[row] * (double)a;
+        }
+    }
+    block_sum_last(d, partial, &ps->ticket[1], &ps->pq);
+}
+
+// x += alpha p, r -= alpha q, rr_new = r . r; the last block sets beta and done
+template <typename V>
+__global__ void __launch_bounds__(256) k_cg_update(int n, V* __restrict__ x, V* __restrict__ r,
+                                                   const V* __restrict__ p, const V* __restrict__ q,
+                                                   double* __restrict__ partial, PpeScalars* ps)
+{
+    if (ps->done) return;
+    const double rr = ps->rr, pq = ps->pq;
+    const V alpha = (V)(rr / pq);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double d = 0.0;
+    if (i < n) {
+        x[i] += alpha * p[i];
+        const V ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        d = (double)ri * (double)ri;
+    }
+    double tot;
+    if (block_sum_last(d, partial, &ps->ticket[2], &tot) && threadIdx.x == 0) {
+        ps->beta = tot / rr;
+        ps->rr = tot;
+        ps->iters += 1;
+        if (!(tot > ps->tol2) || !(pq > 0.0)) ps->done = (pq > 0.0) ? 1 : 2;
+    }
+}
+
+// p = r + beta p
+template <typename V>
+__global__ void __launch_bounds__(256) k_cg_dir(int n, const V* __restrict__ r, V* __restrict__ p,
+                                                const PpeScalars* __restrict__ ps)
+{
+    if (ps->done) return;
+    const V beta = (V)ps->beta;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = r[i] + beta * p[i];
+}
+
+// Outer residual in FP64: r64 = b - A64 x; rr64 = r64 . r64.  Also starts the
+// inner solve A e = r: e = 0, r_in = p = r64 (in the inner precision),
+// rr = r_in . r_in, tol2 = max(tol_in^2 rr, floor2), done = 0.
+template <int G, typename V>
+__global__ void __launch_bounds__(256) k_residual(int n, const int* __restrict__ rp, const int* __restrict__ col,
+                                                  const double* __restrict__ val, const double* __restrict__ b,
+                                                  const double* __restrict__ x, V* __restrict__ e,
+                                                  V* __restrict__ r, V* __restrict__ p,
+                                                  double* __restrict__ partial, double* __restrict__ partial2,
+                                                  PpeScalars* ps, double tol_in2, double floor2)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = t / G, g = t % G;
+    double d64 = 0.0, din = 0.0;
+    if (row < n) {
+        const double a = row_dot<G, double, double, double>(row, g, rp, col, val, x);
+        if (g == 0) {
+            const double ri = b[row] - a;
+            const V rv = (V)ri;
+            e[row] = (V)0;
+            r[row] = rv;
+            p[row] = rv;
+            d64 = ri * ri;
+            din = (double)rv * (double)rv;
+        }
+    }
+    double tot, rin;
+    if (block_sum_last2(d64, din, partial, partial2, &ps->ticket[3], &tot, &rin) && threadIdx.x == 0) {
+        ps->rr64 = tot;
+        ps->rin = rin;
+        ps->rr = rin;
+        ps->rr0 = rin;
+        ps->tol2 = fmax(tol_in2 * rin, floor2);
+        ps->done = rin > ps->tol2 ? 0 : 1;
+        ps->iters = 0;
+    }
+}
+
+template <typename V>
+__global__ void __launch_bounds__(256) k_ir_update(int n, double* __restrict__ x, const V* __restrict__ e)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += (double)e[i];
+}
+
+// pressCorr (MPS_CPU.cpp:734-743): p = max(x, 0), written into the state.
+template <typename Real>
+__global__ void __launch_bounds__(256) k_press_corr(int n, const double* __restrict__ x, Real* __restrict__ p)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (Real)(x[i] > 0.0 ? x[i] : 0.0);
+}
+
+// ------------------------------------------------------------- launchers ---
+static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+constexpr int kG = 16;
+
+int ppe_partial_blocks(int n) { return cdiv((long)n * kG, 256) + cdiv(n, 256) + 8; }
+
+template <typename Real>
+void launch_ppe_rows(int n, StateView<Real> s, const DevScalars* sc, double beta, double btmp_over,
+                     int x0_kind, const double* x0_ext, unsigned char* bc, double* b, double* x,
+                     double* partial, PpeScalars* ps, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ppe_rows<Real>, dim3(cdiv(n, 256)), dim3(256), 0, st, n, s, sc, beta,
+                       btmp_over, x0_kind, x0_ext, bc, b, x, partial, ps);
+}
+
+template <typename Real>
+void launch_ppe_assemble(const Grid& g, const int* cs, StateView<Real> s, const Thresh<Real>& th,
+                         const unsigned char* bc, const DevScalars* sc, double aii_over, int fill,
+                         int* row_ptr, int* col, float* v32, double* v64, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ppe_assemble<Real>, dim3(xcd_grid(g.ncells)), dim3(64), 0, st, g, cs, s, th, bc,
+                       sc, aii_over, fill, row_ptr, col, v32, v64);
+}
+
+template <typename MV, typename V>
+void launch_cg_iter(int n, const int* rp, const int* col, const MV* val, V* x, V* r, V* p, V* q,
+                    double* partial, PpeScalars* ps, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_spmv_dot<kG, MV, V>), dim3(cdiv((long)n * kG, 256)), dim3(256), 0, st, n, rp, col,
+                       val, p, q, partial, ps);
+    hipLaunchKernelGGL(k_cg_update<V>, dim3(cdiv(n, 256)), dim3(256), 0, st, n, x, r, p, q, partial, ps);
+    hipLaunchKernelGGL(k_cg_dir<V>, dim3(cdiv(n, 256)), dim3(256), 0, st, n, r, p, ps);
+}
+
+template <typename V>
+void launch_residual(int n, const int* rp, const int* col, const double* val, const double* b,
+                     const double* x, V* e, V* r, V* p, double* partial, double* partial2,
+                     PpeScalars* ps, double tol_in2, double floor2, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_residual<kG, V>), dim3(cdiv((long)n * kG, 256)), dim3(256), 0, st, n, rp, col, val,
+                       b, x, e, r, p, partial, partial2, ps, tol_in2, floor2);
+}
+
+template <typename V>
+void launch_ir_update(int n, double* x, const V* e, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_ir_update<V>, dim3(cdiv(n, 256)), dim3(256), 0, st, n, x, e);
+}
+
+template <typename Real>
+void launch_press_corr(int n, const double* x, Real* p, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_press_corr<Real>, dim3(cdiv(n, 256)), dim3(256), 0, st, n, x, p);
+}
+
+template void launch_ppe_rows<float>(int, StateView<float>, const DevScalars*, double, double, int,
+                                     const double*, unsigned char*, double*, double*, double*,
+                                     PpeScalars*, hipStream_t);
+template void launch_ppe_rows<double>(int, StateView<double>, const DevScalars*, double, double, int,
+                                      const double*, unsigned char*, double*, double*, double*,
+                                      PpeScalars*, hipStream_t);
+template void launch_ppe_assemble<float>(const Grid&, const int*, StateView<float>, const Thresh<float>&,
+                                         const unsigned char*, const DevScalars*, double, int, int*,
+                                         int*, float*, double*, hipStream_t);
+template void launch_ppe_assemble<double>(const Grid&, const int*, StateView<double>,
+                                          const Thresh<double>&, const unsigned char*,
+                                          const DevScalars*, double, int, int*, int*, float*, double*,
+                                          hipStream_t);
+template void launch_cg_iter<float, float>(int, const int*, const int*, const float*, float*, float*,
+                                           float*, float*, double*, PpeScalars*, hipStream_t);
+template void launch_cg_iter<double, double>(int, const int*, const int*, const double*, double*,
+                                             double*, double*, double*, double*, PpeScalars*,
+                                             hipStream_t);
+template void launch_residual<float>(int, const int*, const int*, const double*, const double*,
+                                     const double*, float*, float*, float*, double*, double*,
+                                     PpeScalars*, double, double, hipStream_t);
+template void launch_residual<double>(int, const int*, const int*, const double*, const double*,
+                                      const double*, double*, double*, double*, double*, double*,
+                                      PpeScalars*, double, double, hipStream_t);
+template void launch_ir_update<float>(int, double*, const float*, hipStream_t);
+template void launch_ir_update<double>(int, double*, const double*, hipStream_t);
+template void launch_press_corr<float>(int, const double*, float*, hipStream_t);
+template void launch_press_corr<double>(int, const double*, double*, hipStream_t);
+
+}  // namespace mps

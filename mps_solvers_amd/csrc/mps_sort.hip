@@ -218,6 +218,11 @@ void launch_radix_sort(int n, int key_bits, uint32_t* k0, uint32_t* v0, uint32_t
     *kres = k0; *vres = v0;
 }
 
+void launch_scan_excl(uint32_t* a, int m, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, a, m);
+}
+
 void launch_cell_bounds(const uint32_t* keys, int n, int ncells, int* cs, hipStream_t st)
 {
     hipLaunchKernelGGL(k_cell_bounds, dim3(cdiv(n + 1, 256)), dim3(256), 0, st, keys, n, ncells, cs);

@@ -2,7 +2,7 @@
 
 Bars (per quantity, normwise: max |gpu - oracle| / max |oracle|):
   * permutations, cell_start, neighbour lists, normals, collision hit counts: bit-exact;
-  * FP32 path (precision 0): 1e-4 on every operator output, n within 1e-9;
+  * FP32 path (precision 0): 1e-4 on every operator output (positions: 4 ulp), n within 1e-12;
   * FP64 path (precision 1): 1e-6 (observed ~1e-13).
 Every operator is driven from an identical sorted state read back from the
 device, so the oracle and the kernels see the same inputs.
@@ -17,6 +17,18 @@ from scenes import (gpu_params, normwise, oracle_grid, oracle_params, oracle_vie
 pytestmark = pytest.mark.gpu
 
 TOL = {0: 1e-4, 1: 1e-6}
+
+
+def _pos_ok(out_pos, o, ref, precision):
+    """Positions: FP64 path normwise on the displacement; FP32 path within
+    4 ulp of the largest coordinate (a displacement dt * du can sit below the
+    FP32 resolution of the coordinate itself)."""
+    for a, k in enumerate("xyz"):
+        if precision == 1:
+            assert normwise(out_pos[:, a] - o[k], ref[k] - o[k]) < TOL[1], k
+        else:
+            ulp = np.spacing(np.float32(np.max(np.abs(ref[k]))))
+            assert np.max(np.abs(out_pos[:, a] - ref[k])) <= 4 * ulp, k
 
 
 def _solver(precision, seed=3, **kw):
@@ -133,8 +145,7 @@ def test_viscosity_operator(precision):
     for a, k in enumerate("uvw"):
         du, du_o = out["vel"][:, a] - o[k], ref[k] - o[k]
         assert normwise(du, du_o) < TOL[precision], k
-    for a, k in enumerate("xyz"):
-        assert normwise(out["pos"][:, a] - o[k], ref[k] - o[k]) < TOL[precision], k
+    _pos_ok(out["pos"], o, ref, precision)
     # viscous part vs gravity-only: the sum really contributed
     assert np.max(np.abs(ref["u"] - o["u"])) > 1e-4
 
@@ -155,6 +166,7 @@ def test_collision_operator(precision):
     assert int(s.scalars()["ncol"]) == ncol
     for a, k in enumerate("uvw"):
         assert normwise(out["vel"][:, a] - o[k], ref[k] - o[k]) < TOL[precision], k
+    _pos_ok(out["pos"], o, ref, precision)
 
 
 @pytest.mark.parametrize("precision", [0, 1])
@@ -172,8 +184,7 @@ def test_pressure_gradient_operator(precision):
     out = s.state()
     for a, k in enumerate("uvw"):
         assert normwise(out["vel"][:, a] - o[k], ref[k] - o[k]) < TOL[precision], k
-    for a, k in enumerate("xyz"):
-        assert normwise(out["pos"][:, a] - o[k], ref[k] - o[k]) < TOL[precision], k
+    _pos_ok(out["pos"], o, ref, precision)
 
 
 @pytest.mark.parametrize("precision", [0, 1])
