@@ -212,7 +212,11 @@ __global__ void __launch_bounds__(256) k_cg_update(int n, V* __restrict__ x, V* 
         ps->beta = tot / rr;
         ps->rr = tot;
         ps->iters += 1;
-        if (!(tot > ps->tol2) || !(pq > 0.0)) ps->done = (pq > 0.0) ? 1 : 2;
+        // the MPS Poisson matrix is negative definite: p.Ap < 0 is normal; only
+        // a vanishing or non-finite p.Ap is a breakdown
+        const bool ok = fabs(pq) > 0.0 && isfinite(pq) && isfinite(tot);
+        if (!ok) ps->done = 2;
+        else if (!(tot > ps->tol2)) ps->done = 1;
     }
 }
 

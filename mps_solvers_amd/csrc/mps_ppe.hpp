@@ -87,6 +87,6 @@ template <typename V>
 void launch_ir_update(int n, double* x, const V* e, hipStream_t st);
 template <typename Real>
 void launch_press_corr(int n, const double* x, Real* p, hipStream_t st);
-void launch_scan_excl(uint32_t* a, int m, hipStream_t st);
+void launch_scan_excl(uint32_t* a, int m, uint32_t* tiles, hipStream_t st);
 
 }  // namespace mps

@@ -117,7 +117,7 @@ static int assemble_t(mps_ctx* c, int n, double alpha, long* nnz)
     launch_ppe_assemble<Real>(c->g, c->cs, s, th, B->bc, c->sc, aii_over, 0, B->rp, nullptr, nullptr,
                               nullptr, c->st);
     HIPOK(hipMemsetAsync(B->rp + n, 0, 4, c->st));
-    launch_scan_excl((uint32_t*)B->rp, n + 1, c->st);
+    launch_scan_excl((uint32_t*)B->rp, n + 1, (uint32_t*)B->partial, c->st);
     int tot = 0;
     HIPOK(hipMemcpyAsync(&tot, B->rp + n, 4, hipMemcpyDeviceToHost, c->st));
     HIPOK(hipStreamSynchronize(c->st));

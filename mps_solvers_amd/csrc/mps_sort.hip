@@ -5,6 +5,8 @@
 // The reference hands out slots with a serial counting sort that is stable in
 // the current order.  A stable LSD radix sort over the cell key with the
 // current index as payload yields exactly that permutation.
+#include <algorithm>
+
 #include "mps_device.hpp"
 #include "mps_kernels.hpp"
 
@@ -77,25 +79,80 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const uint32_t* __res
     hist[(size_t)tid * gridDim.x + blockIdx.x] = h[tid];
 }
 
-// In-place exclusive scan of m counters by one 1024-thread block.
-__global__ void __launch_bounds__(1024) k_scan_excl(uint32_t* __restrict__ a, int m)
+// Exclusive scan of m counters in three launches: per-tile totals, a scan of
+// the tile totals by one block, then each tile rescanned with its offset.
+constexpr int SC_THREADS = 256;
+constexpr int SC_ITEMS = 16;
+constexpr int SC_TILE = SC_THREADS * SC_ITEMS;
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* total)
 {
-    __shared__ uint32_t part[1024];
-    const int tid = threadIdx.x;
-    const int per = (m + 1023) / 1024;
-    const int s0 = min(tid * per, m), s1 = min(s0 + per, m);
-    uint32_t s = 0;
-    for (int i = s0; i < s1; i++) s += a[i];
-    part[tid] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t t = tid >= off ? part[tid - off] : 0u;
-        __syncthreads();
-        part[tid] += t;
-        __syncthreads();
+    __shared__ uint32_t ws[SC_THREADS / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t t = __shfl_up(inc, d);
+        if (lane >= d) inc += t;
     }
-    uint32_t run = part[tid] - s;
-    for (int i = s0; i < s1; i++) { uint32_t t = a[i]; a[i] = run; run += t; }
+    if (lane == 63) ws[w] = inc;
+    __syncthreads();
+    uint32_t off = 0, tot = 0;
+    for (int k = 0; k < SC_THREADS / 64; k++) {
+        if (k < w) off += ws[k];
+        tot += ws[k];
+    }
+    __syncthreads();
+    *total = tot;
+    return off + inc - v;
+}
+
+__global__ void __launch_bounds__(SC_THREADS) k_scan_tiles(const uint32_t* __restrict__ a, int m,
+                                                           uint32_t* __restrict__ tiles)
+{
+    const int base = blockIdx.x * SC_TILE;
+    uint32_t s = 0;
+    for (int k = 0; k < SC_ITEMS; k++) {
+        const int i = base + k * SC_THREADS + threadIdx.x;
+        if (i < m) s += a[i];
+    }
+    uint32_t tot;
+    block_excl_scan(s, &tot);
+    if (threadIdx.x == 0) tiles[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(SC_THREADS) k_scan_top(uint32_t* __restrict__ tiles, int nt)
+{
+    uint32_t carry = 0;
+    for (int base = 0; base < nt; base += SC_THREADS) {
+        const int i = base + threadIdx.x;
+        const uint32_t v = i < nt ? tiles[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan(v, &tot);
+        if (i < nt) tiles[i] = carry + ex;
+        carry += tot;
+    }
+}
+
+// each thread owns SC_ITEMS consecutive counters of the tile
+__global__ void __launch_bounds__(SC_THREADS) k_scan_apply(uint32_t* __restrict__ a, int m,
+                                                           const uint32_t* __restrict__ tiles)
+{
+    const int base = blockIdx.x * SC_TILE + threadIdx.x * SC_ITEMS;
+    uint32_t v[SC_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; k++) {
+        v[k] = base + k < m ? a[base + k] : 0u;
+        s += v[k];
+    }
+    uint32_t tot;
+    uint32_t run = tiles[blockIdx.x] + block_excl_scan(s, &tot);
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; k++) {
+        if (base + k < m) a[base + k] = run;
+        run += v[k];
+    }
 }
 
 // Stable scatter.  Each round covers 256 consecutive keys; within a wave the
@@ -190,7 +247,20 @@ __global__ void __launch_bounds__(256) k_normal_remap(int n, const int* __restri
 // ------------------------------------------------------------- drivers ---
 static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
-int sort_scratch_words(int n) { return 256 * cdiv(n, RS_TILE); }
+// radix histogram (256 digits x blocks) followed by the tile totals of its scan
+int sort_scratch_words(int n)
+{
+    const int m = 256 * cdiv(n, RS_TILE);
+    return m + cdiv(std::max(m, n + 1), SC_TILE) + 64;
+}
+
+static void scan_excl(uint32_t* a, int m, uint32_t* tiles, hipStream_t st)
+{
+    const int nt = cdiv(m, SC_TILE);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SC_THREADS), 0, st, a, m, tiles);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SC_THREADS), 0, st, tiles, nt);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(SC_THREADS), 0, st, a, m, tiles);
+}
 
 template <typename Real>
 void launch_hash(const Grid& g, int n, StateView<Real> s, uint32_t* key, uint32_t* val, int* nexcl,
@@ -209,7 +279,7 @@ void launch_radix_sort(int n, int key_bits, uint32_t* k0, uint32_t* v0, uint32_t
     const int nb = cdiv(n, RS_TILE);
     for (int shift = 0; shift < key_bits; shift += 8) {
         hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(RS_THREADS), 0, st, k0, n, shift, hist);
-        hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, hist, 256 * nb);
+        scan_excl(hist, 256 * nb, hist + 256 * nb, st);
         hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(RS_THREADS), 0, st, k0, v0, k1, v1, n,
                            shift, hist);
         uint32_t* t = k0; k0 = k1; k1 = t;
@@ -218,10 +288,7 @@ void launch_radix_sort(int n, int key_bits, uint32_t* k0, uint32_t* v0, uint32_t
     *kres = k0; *vres = v0;
 }
 
-void launch_scan_excl(uint32_t* a, int m, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, a, m);
-}
+void launch_scan_excl(uint32_t* a, int m, uint32_t* tiles, hipStream_t st) { scan_excl(a, m, tiles, st); }
 
 void launch_cell_bounds(const uint32_t* keys, int n, int ncells, int* cs, hipStream_t st)
 {
