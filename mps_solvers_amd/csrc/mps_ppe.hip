@@ -17,7 +17,7 @@
 #include "mps_device.hpp"
 #include "mps_kernels.hpp"
 #include "mps_ppe.hpp"
-#include "mps_stage.hpp"
+#include "mps_tile.hpp"
 
 namespace mps {
 
@@ -51,6 +51,9 @@ __global__ void __launch_bounds__(256) k_ppe_rows(int n, StateView<Real> s, cons
 }
 
 // ------------------------------------------------------------- assembly ---
+// The 8 lanes of a row walk its candidate list with stride 8; in each round
+// the accepted entries of the 8 lanes are compacted in lane order, so the
+// row's columns are written in ascending order (the list ascends).
 template <typename Real>
 struct OpPpe {
     const Real *x, *y, *z;
@@ -58,10 +61,9 @@ struct OpPpe {
     const unsigned char* bcv;
     Real tl;
     double re;
-    int i, fill;
+    int i, sub, fill;
     bool active;
     Real xi, yi, zi;
-    int cnt;
     double diag;
     long kpos, kdiag;
     int* col;
@@ -73,67 +75,86 @@ struct OpPpe {
         c.x = x[j]; c.y = y[j]; c.z = z[j]; c.j = j;
         c.t = type[j] | (bcv[j] ? 16 : 0);
     }
-    __device__ void process(const Cand<Real>* buf, int nc)
+    __device__ void process(const Cand<Real>* S, const uint16_t* L, int len)
     {
-        for (int k = 0; k < nc; k++) {
-            const Real cx = buf[k].x, cy = buf[k].y, cz = buf[k].z;
-            const int cj = buf[k].j, ct = buf[k].t;
-            if (!active) continue;
-            if (cj == i) {
-                if (fill) { kdiag = kpos; col[kpos] = i; }
-                kpos++;
-                continue;
+        const int lane = threadIdx.x & 63;
+        for (int k0 = 0; k0 < len; k0 += TK) {
+            const int k = k0 + sub;
+            bool acc = false, isdiag = false;
+            double w = 0.0;
+            int cj = -1;
+            if (active && k < len) {
+                const Cand<Real>& c = S[L[k]];
+                cj = c.j;
+                if (cj == i) {
+                    isdiag = true;
+                    acc = true;
+                } else if ((c.t & 15) != 2) {
+                    const Real rr = rr3(c.x - xi, c.y - yi, c.z - zi);
+                    if (rr < tl) {
+                        const double dx = (double)c.x - (double)xi, dy = (double)c.y - (double)yi,
+                                     dz = (double)c.z - (double)zi;
+                        w = __builtin_fma(re, rsq(dx * dx + dy * dy + dz * dz), -1.0);
+                        diag -= w;
+                        acc = !(c.t & 16);   // Dirichlet neighbour: A_ij = 0
+                    }
+                }
             }
-            if ((ct & 15) == 2) continue;
-            const Real rr = rr3(cx - xi, cy - yi, cz - zi);
-            if (!(rr < tl)) continue;
-            const double dx = (double)cx - (double)xi, dy = (double)cy - (double)yi,
-                         dz = (double)cz - (double)zi;
-            const double w = __builtin_fma(re, rsq(dx * dx + dy * dy + dz * dz), -1.0);
-            diag -= w;
-            if (ct & 16) continue;   // Dirichlet neighbour: A_ij = 0
-            if (fill) { col[kpos] = cj; v64[kpos] = w; v32[kpos] = (float)w; }
-            kpos++;
+            const uint64_t m = ballot(acc);
+            const uint32_t gm = (uint32_t)(m >> (lane & ~7)) & 0xFFu;
+            const int pre = __popc(gm & ((1u << sub) - 1u));
+            if (acc && fill) {
+                const long pos = kpos + pre;
+                col[pos] = cj;
+                if (isdiag) kdiag = pos;
+                else { v64[pos] = w; v32[pos] = (float)w; }
+            }
+            kpos += __popc(gm);
         }
     }
 };
 
 template <typename Real>
-__global__ void __launch_bounds__(64) k_ppe_assemble(Grid g, const int* __restrict__ cs, StateView<Real> s,
+__global__ void __launch_bounds__(TB) k_ppe_assemble(Grid g, const int* __restrict__ cs, StateView<Real> s,
                                                      Thresh<Real> th, const unsigned char* __restrict__ bc,
                                                      const DevScalars* __restrict__ sc, double aii_over,
                                                      int fill, int* __restrict__ row_ptr,
                                                      int* __restrict__ col, float* __restrict__ v32,
                                                      double* __restrict__ v64)
 {
-    __shared__ Cand<Real> buf[Cap<Real>::v];
+    __shared__ TileShared<Real> sh;
     const int cell = xcd_cell(g.ncells);
     if (cell >= g.ncells) return;
     const int s0 = cs[cell], s1 = cs[cell + 1];
     // rho lambda alpha n0 / (2 d dt^2)
     const double dt = sc->dt;
     const double aii = aii_over * sc->lambda * sc->n0 / (dt * dt);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, sub = lane & 7;
     OpPpe<Real> op;
     op.x = s.x; op.y = s.y; op.z = s.z; op.type = s.type; op.bcv = bc;
-    op.tl = th.t_lap; op.re = (double)th.re_lap; op.fill = fill;
+    op.tl = th.t_lap; op.re = (double)th.re_lap; op.fill = fill; op.sub = sub;
     op.col = col; op.v32 = v32; op.v64 = v64;
     const Real R2 = filt_r2(th.re_lap);
-    for (int c0 = s0; c0 < s1; c0 += WAVE) {
-        const int i = c0 + (int)threadIdx.x;
+    for (int c0 = s0; c0 < s1; c0 += TW * 8) {
+        const int i = c0 + wave * 8 + (lane >> 3);
         const bool in = i < s1;
         op.i = i;
         op.active = false;
         op.diag = 0.0;
         op.kpos = fill && in ? row_ptr[i] : 0;
         op.kdiag = -1;
+        op.xi = op.yi = op.zi = (Real)0;
         if (in) {
             op.xi = s.x[i]; op.yi = s.y[i]; op.zi = s.z[i];
             op.active = !bc[i] && s.type[i] != 2;
         }
-        Real lo[3], hi[3];
-        wave_box(op.active, op.xi, op.yi, op.zi, lo, hi);
-        if (ballot(op.active)) stage(g, cs, cell, lo, hi, R2, buf, op);
-        if (in) {
+        Real wlo[3], whi[3], blo[3], bhi[3];
+        bool tact;
+        if (tile_boxes(sh, op.active, op.xi, op.yi, op.zi, wlo, whi, blo, bhi, tact))
+            tile_stage(g, cs, cell, sh, blo, bhi, wlo, whi, R2, tact, op);
+        const double diag = sub_sum(op.diag);
+        const long kdiag = sub_max(op.kdiag);
+        if (in && sub == 0) {
             if (!op.active) {
                 if (fill) {
                     const long k = row_ptr[i];
@@ -142,14 +163,13 @@ __global__ void __launch_bounds__(64) k_ppe_assemble(Grid g, const int* __restri
                     row_ptr[i] = 1;
                 }
             } else if (fill) {
-                const double d = op.diag - aii;
-                v64[op.kdiag] = d;
-                v32[op.kdiag] = (float)d;
+                const double d = diag - aii;
+                v64[kdiag] = d;
+                v32[kdiag] = (float)d;
             } else {
                 row_ptr[i] = (int)op.kpos;
             }
         }
-        __syncthreads();
     }
 }
 
@@ -304,7 +324,7 @@ void launch_ppe_assemble(const Grid& g, const int* cs, StateView<Real> s, const 
                          const unsigned char* bc, const DevScalars* sc, double aii_over, int fill,
                          int* row_ptr, int* col, float* v32, double* v64, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_ppe_assemble<Real>, dim3(xcd_grid(g.ncells)), dim3(64), 0, st, g, cs, s, th, bc,
+    hipLaunchKernelGGL(k_ppe_assemble<Real>, dim3(xcd_grid(g.ncells)), dim3(TB), 0, st, g, cs, s, th, bc,
                        sc, aii_over, fill, row_ptr, col, v32, v64);
 }
 
