@@ -166,6 +166,34 @@ int mps_ppe_press_corr(mps_ctx* c);
 int mps_step_semi(mps_ctx* c, int nsteps, double alpha, int x0_kind, double tol, int* iters_total,
                   double* solve_ms);
 
+/* --------------------------------------------- learned CG warm start (no reference counterpart) */
+
+/* Per-particle MLP 16 -> 64 -> 64 -> 64 -> 1 (SiLU), BF16 MFMA with FP32
+ * accumulation, FP32 master parameters (9,473 values, flat order W1[16x64],
+ * b1, W2[64x64], b2, W3[64x64], b3, W4[64], b4).  The guess is
+ * x0 = p_prev + P_s * y with P_s = rho |g| 64 dp.  These entry points have no
+ * counterpart in the reference; they feed mps_ppe_solve(x0_kind = 3). */
+int mps_mlp_nparams(void);
+int mps_mlp_init(mps_ctx* c, unsigned long long seed);
+int mps_mlp_set_params(mps_ctx* c, const float* theta);
+int mps_mlp_get_params(mps_ctx* c, float* theta);
+/* y = MLP(X) for n caller feature rows (n x 16). */
+int mps_mlp_forward(mps_ctx* c, const float* X, int n, float* y);
+/* Gradient (9,473) and value of mean((MLP(X) - t)^2) over n caller samples. */
+int mps_mlp_grad(mps_ctx* c, const float* X, const float* t, int n, float* grad, double* loss);
+/* 16 features per row of the assembled Poisson system (F may be NULL). */
+int mps_mlp_features(mps_ctx* c, float* F, int* nrows);
+/* Features + inference into the solver's guess buffer. */
+int mps_mlp_infer(mps_ctx* c);
+/* When on, mps_step_semi appends (features, converged correction) of every
+ * solve to a device-resident training set. */
+int mps_mlp_set_record(mps_ctx* c, int on);
+int mps_mlp_dataset(mps_ctx* c, long* count, float* X, float* T);
+int mps_mlp_dataset_clear(mps_ctx* c);
+/* iters Adam steps (lr) on minibatches of the training set. */
+int mps_mlp_train(mps_ctx* c, int iters, int batch, double lr, unsigned long long seed,
+                  double* loss_first, double* loss_last);
+
 #ifdef __cplusplus
 }
 #endif

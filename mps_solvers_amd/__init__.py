@@ -94,6 +94,18 @@ def lib():
         L.mps_ppe_solve.argtypes = [vp, C.c_int, C.c_int, D, C.c_double, C.c_int, I, D]
         L.mps_ppe_press_corr.argtypes = [vp]
         L.mps_step_semi.argtypes = [vp, C.c_int, C.c_double, C.c_int, C.c_double, I, D]
+        F = C.POINTER(C.c_float)
+        L.mps_mlp_init.argtypes = [vp, C.c_ulonglong]
+        L.mps_mlp_set_params.argtypes = [vp, F]
+        L.mps_mlp_get_params.argtypes = [vp, F]
+        L.mps_mlp_forward.argtypes = [vp, F, C.c_int, F]
+        L.mps_mlp_grad.argtypes = [vp, F, F, C.c_int, F, D]
+        L.mps_mlp_features.argtypes = [vp, F, I]
+        L.mps_mlp_infer.argtypes = [vp]
+        L.mps_mlp_set_record.argtypes = [vp, C.c_int]
+        L.mps_mlp_dataset.argtypes = [vp, C.POINTER(C.c_long), F, F]
+        L.mps_mlp_dataset_clear.argtypes = [vp]
+        L.mps_mlp_train.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_ulonglong, D, D]
         _lib = L
     return _lib
 
@@ -109,6 +121,10 @@ def _dp(a):
 
 def _ip(a):
     return None if a is None else a.ctypes.data_as(I)
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
 
 
 def params(**kw) -> Params:
@@ -321,6 +337,64 @@ class Solver:
         _check(lib().mps_step_semi(self.h, nsteps, float(alpha), int(x0_kind), float(tol),
                                    C.byref(it), C.byref(ms)), "mps_step_semi")
         return it.value, ms.value
+
+    # ------------------------------------------- learned warm start (MLP)
+    def mlp_init(self, seed=0):
+        _check(lib().mps_mlp_init(self.h, seed), "mps_mlp_init")
+
+    def mlp_set_params(self, theta):
+        th = np.ascontiguousarray(theta, np.float32)
+        _check(lib().mps_mlp_set_params(self.h, _fp(th)), "mps_mlp_set_params")
+
+    def mlp_get_params(self):
+        th = np.zeros(lib().mps_mlp_nparams(), np.float32)
+        _check(lib().mps_mlp_get_params(self.h, _fp(th)), "mps_mlp_get_params")
+        return th
+
+    def mlp_forward(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.zeros(len(X), np.float32)
+        _check(lib().mps_mlp_forward(self.h, _fp(X), len(X), _fp(y)), "mps_mlp_forward")
+        return y
+
+    def mlp_grad(self, X, t):
+        X = np.ascontiguousarray(X, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        g = np.zeros(lib().mps_mlp_nparams(), np.float32)
+        loss = C.c_double()
+        _check(lib().mps_mlp_grad(self.h, _fp(X), _fp(t), len(X), _fp(g), C.byref(loss)), "mps_mlp_grad")
+        return loss.value, g
+
+    def mlp_features(self):
+        n = C.c_int()
+        _check(lib().mps_mlp_features(self.h, None, C.byref(n)), "mps_mlp_features")
+        F = np.zeros((n.value, 16), np.float32)
+        _check(lib().mps_mlp_features(self.h, _fp(F), C.byref(n)), "mps_mlp_features")
+        return F
+
+    def mlp_infer(self):
+        _check(lib().mps_mlp_infer(self.h), "mps_mlp_infer")
+
+    def mlp_set_record(self, on=True):
+        _check(lib().mps_mlp_set_record(self.h, int(on)), "mps_mlp_set_record")
+
+    def mlp_dataset(self):
+        n = C.c_long()
+        _check(lib().mps_mlp_dataset(self.h, C.byref(n), None, None), "mps_mlp_dataset")
+        X = np.zeros((n.value, 16), np.float32)
+        T = np.zeros(n.value, np.float32)
+        if n.value:
+            _check(lib().mps_mlp_dataset(self.h, C.byref(n), _fp(X), _fp(T)), "mps_mlp_dataset")
+        return X, T
+
+    def mlp_dataset_clear(self):
+        _check(lib().mps_mlp_dataset_clear(self.h), "mps_mlp_dataset_clear")
+
+    def mlp_train(self, iters, batch=4096, lr=1e-3, seed=0):
+        l0, l1 = C.c_double(), C.c_double()
+        _check(lib().mps_mlp_train(self.h, iters, batch, lr, seed, C.byref(l0), C.byref(l1)),
+               "mps_mlp_train")
+        return l0.value, l1.value
 
     def enable_timing(self, on=True):
         _check(lib().mps_enable_timing(self.h, int(on)), "mps_enable_timing")

@@ -250,6 +250,24 @@ extern "C" int mps_ppe_get_x(mps_ctx* c, double* x)
     return mps_ppe_get(c, nullptr, nullptr, nullptr, nullptr, x, nullptr);
 }
 
+struct PpeView {
+    int n;
+    const int* rp;
+    const int* col;
+    const double* v64;
+    const double* b;
+    const double* x;
+    const unsigned char* bc;
+};
+
+int mps_ppe_view(mps_ctx* c, PpeView* v)
+{
+    if (!c->ppe || !c->ppe->assembled) return MPS_ERR_STATE;
+    PpeBuffers* B = c->ppe;
+    v->n = B->n; v->rp = B->rp; v->col = B->col; v->v64 = B->v64; v->b = B->b; v->x = B->x; v->bc = B->bc;
+    return MPS_OK;
+}
+
 double* mps_ppe_guess_buffer(mps_ctx* c, int n)
 {
     if (ensure_rows(c, n)) return nullptr;
@@ -284,7 +302,10 @@ extern "C" int mps_stage_pnd(mps_ctx* c)
     return MPS_OK;
 }
 
-int mps_mlp_guess(mps_ctx* c);   // mps_mlp_capi: features + inference into the guess buffer
+int mps_mlp_guess(mps_ctx* c);      // mps_mlp.hip: features + inference into the guess buffer
+int mps_mlp_prepare(mps_ctx* c);    // mps_mlp.hip: features of the assembled system
+int mps_mlp_record(mps_ctx* c);     // mps_mlp.hip: append (features, solution) to the dataset
+int mps_mlp_recording(mps_ctx* c);
 
 // MPS_CPU_EXPL::step (MPS_CAL_v4.0/MPS_CPU_EXPL.cpp:79-135), two-pass operators:
 // dt, [re-bin], viscosity + gravity, collision, n*, Poisson build + CG solve,
@@ -309,7 +330,12 @@ extern "C" int mps_step_semi(mps_ctx* c, int nsteps, double alpha, int x0_kind, 
         if ((rc = mps_ctx_stage(c, 5))) break;
         long nnz = 0;
         if ((rc = mps_ppe_assemble(c, alpha, &nnz))) break;
-        if (x0_kind == 3 && (rc = mps_mlp_guess(c))) break;
+        const bool rec = mps_mlp_recording(c) != 0;
+        if (x0_kind == 3) {
+            if ((rc = mps_mlp_guess(c))) break;
+        } else if (rec && (rc = mps_mlp_prepare(c))) {
+            break;
+        }
         hipEvent_t e0, e1;
         hipEventCreate(&e0);
         hipEventCreate(&e1);
@@ -326,6 +352,7 @@ extern "C" int mps_step_semi(mps_ctx* c, int nsteps, double alpha, int x0_kind, 
         ms += t;
         total += it;
         if (rc) break;
+        if (rec && (rc = mps_mlp_record(c))) break;
         if ((rc = press_corr(c))) break;
         c->kc.gradp_skip_dummy = 1;
         rc = mps_ctx_stage(c, 4);
