@@ -53,8 +53,16 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # command-line driver linked against the library (rpath: its own folder)
+    exe = os.path.join(HERE, "mps_cal")
+    cmd = ["g++", "-O2", "-std=c++17", "-o", exe + ".tmp", os.path.join(CSRC, "mps_cal_main.cpp"),
+           "-L" + HERE, "-lmps_amd", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(exe + ".tmp", exe)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {exe}")
     return LIB
 
 
