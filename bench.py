@@ -74,6 +74,82 @@ def cpu_baseline(w, nb2, nb1, pos, budget_s=20.0):
                       f"{el:.1f} s on {threads} OpenMP threads (oracle/mps_oracle.c, FP64 sums)"}
 
 
+PPE_SCENES = {
+    # config 2: same scene family at ~0.5M fluid (64 x 64 x 128), re_lap = 3.1 l0
+    "eval": dict(fluid=(64, 64, 128), inner=(96, 64, 134), dp=0.0025, seed=2),
+    # training runs for the learned guess (seeded dam breaks of ~100k-200k fluid)
+    "train": [dict(fluid=(40, 40, 64), inner=(60, 40, 70), dp=0.0025, seed=11),
+              dict(fluid=(48, 48, 80), inner=(72, 48, 86), dp=0.0025, seed=12)],
+}
+
+
+def ppe_params(mps, dp):
+    return mps.params(dim=3, precision=0, rho=1000.0, nu=1e-6, dp=dp, re_n_mult=2.1,
+                      re_lap_mult=3.1, beta=0.97, cs=20.0, cfl=0.1, col_mult=0.8, col_rate=2.0,
+                      dt_max=2.5e-4, dt_min_mult=1e-3, max_dis=dp * 0.1, g=(0.0, 0.0, -9.81))
+
+
+def ppe_bench(mps, spin_steps=20, train_iters=2000, log=print):
+    """Pressure Poisson solve to |b - Ax| <= 1e-8 |b| (FP64 residual) on the
+    config-2 scene: mixed-precision CG from zero (plain), from the previous
+    pressure, and from the learned guess (inference time included); full FP64
+    CG from zero for reference."""
+    t0 = time.perf_counter()
+    Xs, Ts = [], []
+    for sc in PPE_SCENES["train"]:
+        nb2, nb1, pos = mps.dambreak(sc["fluid"], sc["inner"], 2, 2, sc["dp"], 0.01, sc["seed"])
+        s = mps.Solver(ppe_params(mps, sc["dp"]), nb2, nb1, pos)
+        s.mlp_init(0)
+        s.step_semi(10, alpha=0.0, x0_kind=1, tol=1e-8)
+        s.mlp_set_record(True)
+        s.step_semi(spin_steps, alpha=0.0, x0_kind=1, tol=1e-8)
+        X, T = s.mlp_dataset()
+        Xs.append(X)
+        Ts.append(T)
+        s.close()
+        log(f"[bench] recorded {len(X)} samples from a {len(pos)}-particle dam break")
+    sc = PPE_SCENES["eval"]
+    nb2, nb1, pos = mps.dambreak(sc["fluid"], sc["inner"], 2, 2, sc["dp"], 0.01, sc["seed"])
+    s = mps.Solver(ppe_params(mps, sc["dp"]), nb2, nb1, pos)
+    s.mlp_init(0)
+    s.mlp_dataset_append(np.concatenate(Xs), np.concatenate(Ts))
+    t1 = time.perf_counter()
+    l0, l1 = s.mlp_train(train_iters, batch=4096, lr=1e-3, seed=0)
+    train_s = time.perf_counter() - t1
+    log(f"[bench] trained {train_iters} Adam steps in {train_s:.1f} s: loss {l0:.3e} -> {l1:.3e}")
+    s.step_semi(10 + spin_steps, alpha=0.0, x0_kind=1, tol=1e-8)
+    # the system of the next step, built as mps_step_semi builds it
+    s.stage_dt()
+    s.stage_sort()
+    s.stage_visc()
+    s.stage_collision()
+    s.stage_pnd()
+    nnz = s.ppe_assemble(0.0)
+    n = len(s.ppe_get()["b"])
+    res = {}
+    for name, mixed, kind in (("plain_zero", True, 0), ("prev_step", True, 1), ("learned", True, 3),
+                              ("fp64_zero", False, 0)):
+        best = None
+        for rep in range(3):
+            t2 = time.perf_counter()
+            if kind == 3:
+                s.mlp_infer()
+            it, rel = s.ppe_solve(mixed=mixed, x0_kind=kind, tol=1e-8)
+            ms = 1e3 * (time.perf_counter() - t2)
+            if best is None or ms < best["ms"]:
+                best = {"ms": round(ms, 3), "iters": it, "relres": rel}
+        res[name] = best
+    s.close()
+    return {"workload": "dam break 0.52M fluid (64x64x128, l0=2.5mm), re_lap=3.1 l0, textbook PPE "
+                        f"(alpha=0), after {10 + spin_steps} semi-implicit steps of dt=2.5e-4 s",
+            "rows": n, "nnz": int(nnz), "solves": res,
+            "learned_over_plain_time": res["learned"]["ms"] / res["plain_zero"]["ms"],
+            "learned_over_prev_time": res["learned"]["ms"] / res["prev_step"]["ms"],
+            "train": {"samples": int(sum(len(x) for x in Xs)), "adam_steps": train_iters,
+                      "loss_first": l0, "loss_last": l1, "train_s": round(train_s, 2),
+                      "total_s": round(time.perf_counter() - t0, 2)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +158,7 @@ def main():
     ap.add_argument("--config", type=int, default=1, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-ppe", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the explicit "
@@ -140,6 +217,9 @@ def main():
                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic},
     }
     s.close()
+    if not a.no_ppe:
+        print("[bench] pressure Poisson: plain vs previous-step vs learned guess", file=sys.stderr, flush=True)
+        out["ppe"] = ppe_bench(mps, log=lambda m: print(m, file=sys.stderr, flush=True))
     if not a.no_cpu:
         print("[bench] timing the CPU oracle on a bounded sample", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(w, nb2, nb1, pos, a.cpu_budget)

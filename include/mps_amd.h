@@ -189,6 +189,7 @@ int mps_mlp_infer(mps_ctx* c);
  * solve to a device-resident training set. */
 int mps_mlp_set_record(mps_ctx* c, int on);
 int mps_mlp_dataset(mps_ctx* c, long* count, float* X, float* T);
+int mps_mlp_dataset_append(mps_ctx* c, const float* X, const float* T, long n);
 int mps_mlp_dataset_clear(mps_ctx* c);
 /* iters Adam steps (lr) on minibatches of the training set. */
 int mps_mlp_train(mps_ctx* c, int iters, int batch, double lr, unsigned long long seed,

@@ -105,6 +105,7 @@ def lib():
         L.mps_mlp_set_record.argtypes = [vp, C.c_int]
         L.mps_mlp_dataset.argtypes = [vp, C.POINTER(C.c_long), F, F]
         L.mps_mlp_dataset_clear.argtypes = [vp]
+        L.mps_mlp_dataset_append.argtypes = [vp, F, F, C.c_long]
         L.mps_mlp_train.argtypes = [vp, C.c_int, C.c_int, C.c_double, C.c_ulonglong, D, D]
         _lib = L
     return _lib
@@ -386,6 +387,11 @@ class Solver:
         if n.value:
             _check(lib().mps_mlp_dataset(self.h, C.byref(n), _fp(X), _fp(T)), "mps_mlp_dataset")
         return X, T
+
+    def mlp_dataset_append(self, X, T):
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        _check(lib().mps_mlp_dataset_append(self.h, _fp(X), _fp(T), len(X)), "mps_mlp_dataset_append")
 
     def mlp_dataset_clear(self):
         _check(lib().mps_mlp_dataset_clear(self.h), "mps_mlp_dataset_clear")

@@ -19,7 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libmps_amd.so")
-SOURCES = ["mps_sort.hip", "mps_sweeps.hip", "mps_ppe.hip", "mps_mlp.hip", "mps_capi.hip",
+SOURCES = ["mps_sort.hip", "mps_sweeps.hip", "mps_sweeps_wave.hip", "mps_ppe.hip", "mps_mlp.hip", "mps_capi.hip",
            "mps_ppe_capi.hip", "mps_host.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -483,6 +483,11 @@ extern "C" int mps_create(const mps_params* p, int np, int nb2, int nb1, const d
     k.g[0] = p->g[0]; k.g[1] = p->g[1]; k.g[2] = p->g[2];
     k.dim = p->dim;
     k.gradp_skip_dummy = p->gradp_skip_dummy;
+    {
+        // sweep kernels: MPS_SWEEP=tile selects the eight-particle-tile variant
+        const char* v = getenv("MPS_SWEEP");
+        k.variant = (v && !strcmp(v, "tile")) ? 1 : 0;
+    }
     double fl2 = 0.0, st2 = 0.0;
     for (int i = 0; i < np; i++) {
         double a = vel ? vel[3 * i] * vel[3 * i] + vel[3 * i + 1] * vel[3 * i + 1] + vel[3 * i + 2] * vel[3 * i + 2] : 0.0;

@@ -36,7 +36,24 @@ struct StepConst {
     double umax2_static;   // max |u|^2 over particles that never move
     int dim;
     int gradp_skip_dummy;
+    int variant;   // neighbour-sweep kernels: 0 one wavefront per cell, 1 eight-particle tiles
+    int pad_;
 };
+
+namespace wv {
+template <typename Real>
+void launch_pnd(const Grid& g, const int* cs, StateView<Real> s, const Thresh<Real>& th, const StepConst& k,
+                DevScalars* sc, int all_types, int with_pressure, hipStream_t st);
+template <typename Real>
+void launch_visc(const Grid& g, const int* cs, StateView<Real> a, StateView<Real> b, const Thresh<Real>& th,
+                 const StepConst& k, DevScalars* sc, hipStream_t st);
+template <typename Real>
+void launch_collision(const Grid& g, const int* cs, StateView<Real> a, StateView<Real> b, const Thresh<Real>& th,
+                      const StepConst& k, DevScalars* sc, hipStream_t st);
+template <typename Real>
+void launch_gradp(const Grid& g, const int* cs, StateView<Real> a, StateView<Real> b, const Thresh<Real>& th,
+                  const StepConst& k, DevScalars* sc, hipStream_t st);
+}  // namespace wv
 
 // sort
 int sort_scratch_words(int n);

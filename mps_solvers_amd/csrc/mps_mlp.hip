@@ -797,6 +797,8 @@ extern "C" int mps_mlp_infer(mps_ctx* c)
     return MPS_OK;
 }
 
+static int dataset_reserve(mps_ctx* c, long need);
+
 // Append (features, converged correction) of the last solve to the dataset.
 int mps_mlp_record(mps_ctx* c)
 {
@@ -812,21 +814,7 @@ int mps_mlp_record(mps_ctx* c)
     uint32_t add = 0;
     HIPOK(hipMemcpyAsync(&add, M->keep + n, 4, hipMemcpyDeviceToHost, c->st));
     HIPOK(hipStreamSynchronize(c->st));
-    if (M->count + (long)add > M->cap) {
-        const long ncap = std::max((long)((M->count + add) * 1.5), 1L << 16);
-        float *nx = nullptr, *nt = nullptr;
-        HIPOK(hipMalloc(&nx, 4 * (size_t)ncap * NIN));
-        HIPOK(hipMalloc(&nt, 4 * (size_t)ncap));
-        if (M->count) {
-            HIPOK(hipMemcpy(nx, M->DX, 4 * (size_t)M->count * NIN, hipMemcpyDeviceToDevice));
-            HIPOK(hipMemcpy(nt, M->DT, 4 * (size_t)M->count, hipMemcpyDeviceToDevice));
-        }
-        mfree((void*&)M->DX);
-        mfree((void*&)M->DT);
-        M->DX = nx;
-        M->DT = nt;
-        M->cap = ncap;
-    }
+    if ((rc = dataset_reserve(c, M->count + (long)add))) return rc;
     hipLaunchKernelGGL(k_append, dim3((n + 255) / 256), dim3(256), 0, c->st, n, M->keep, M->ident, M->F, v.x,
                        mlp_ps(c), M->count, M->cap, M->DX, M->DT);
     HIPOK(hipGetLastError());
@@ -842,6 +830,41 @@ extern "C" int mps_mlp_dataset(mps_ctx* c, long* count, float* X, float* T)
     HIPOK(hipStreamSynchronize(c->st));
     if (X) HIPOK(hipMemcpy(X, c->mlp->DX, 4 * (size_t)c->mlp->count * NIN, hipMemcpyDeviceToHost));
     if (T) HIPOK(hipMemcpy(T, c->mlp->DT, 4 * (size_t)c->mlp->count, hipMemcpyDeviceToHost));
+    return MPS_OK;
+}
+
+static int dataset_reserve(mps_ctx* c, long need)
+{
+    MlpBuffers* M = c->mlp;
+    if (need <= M->cap) return MPS_OK;
+    const long ncap = std::max((long)(need * 1.5), 1L << 16);
+    float *nx = nullptr, *nt = nullptr;
+    HIPOK(hipMalloc(&nx, 4 * (size_t)ncap * NIN));
+    HIPOK(hipMalloc(&nt, 4 * (size_t)ncap));
+    if (M->count) {
+        HIPOK(hipMemcpy(nx, M->DX, 4 * (size_t)M->count * NIN, hipMemcpyDeviceToDevice));
+        HIPOK(hipMemcpy(nt, M->DT, 4 * (size_t)M->count, hipMemcpyDeviceToDevice));
+    }
+    mfree((void*&)M->DX);
+    mfree((void*&)M->DT);
+    M->DX = nx;
+    M->DT = nt;
+    M->cap = ncap;
+    return MPS_OK;
+}
+
+// Append caller samples (n x 16 features, n targets) to the training set.
+extern "C" int mps_mlp_dataset_append(mps_ctx* c, const float* X, const float* T, long n)
+{
+    if (!c || !X || !T || n < 0) return MPS_ERR_ARG;
+    int rc = mlp_ensure(c);
+    if (rc) return rc;
+    MlpBuffers* M = c->mlp;
+    if ((rc = dataset_reserve(c, M->count + n))) return rc;
+    HIPOK(hipStreamSynchronize(c->st));
+    HIPOK(hipMemcpy(M->DX + M->count * NIN, X, 4 * (size_t)n * NIN, hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(M->DT + M->count, T, 4 * (size_t)n, hipMemcpyHostToDevice));
+    M->count += n;
     return MPS_OK;
 }
 

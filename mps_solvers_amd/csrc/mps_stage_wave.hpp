@@ -1,0 +1,101 @@
+// mps_stage_wave.hpp -- one-wavefront-per-cell LDS staging of the 27-cell
+// candidate set (the "wave" sweep variant).
+#pragma once
+
+#include <float.h>
+
+#include "mps_device.hpp"
+
+namespace mps {
+namespace wv {
+
+template <typename Real>
+struct alignas(16) Cand {
+    Real x, y, z;
+    int j, t;
+    Real a, b, c;
+};
+
+template <typename Real> struct Cap { static constexpr int v = 512; };
+template <> struct Cap<double> { static constexpr int v = 256; };
+
+template <typename Real> __device__ __forceinline__ Real big();
+template <> __device__ __forceinline__ float big<float>() { return FLT_MAX; }
+template <> __device__ __forceinline__ double big<double>() { return DBL_MAX; }
+
+template <typename Real>
+__device__ __forceinline__ Real box_d2(Real qx, Real qy, Real qz, const Real* lo, const Real* hi)
+{
+    Real dx = fmax(fmax(lo[0] - qx, qx - hi[0]), (Real)0);
+    Real dy = fmax(fmax(lo[1] - qy, qy - hi[1]), (Real)0);
+    Real dz = fmax(fmax(lo[2] - qz, qz - hi[2]), (Real)0);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+struct StageResult { int count; bool overflowed; };
+
+// Stream the stencil of `cell`, keep candidates within sqrt(R2) of the box,
+// and hand full LDS batches to op.process().  The last batch stays in `buf`.
+template <typename Real, class Op>
+__device__ StageResult stage(const Grid& g, const int* __restrict__ cs, int cell, const Real* lo,
+                             const Real* hi, Real R2, Cand<Real>* buf, Op& op)
+{
+    constexpr int CAP = Cap<Real>::v;
+    const int lane = threadIdx.x;
+    int cx, cy, cz;
+    cell_xyz(g, cell, cx, cy, cz);
+    const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < g.dx ? cx + 1 : g.dx - 1;
+    int count = 0;
+    bool over = false;
+    for (int dz = -1; dz <= 1; dz++) {
+        const int zz = cz + dz;
+        if (zz < 0 || zz >= g.dz) continue;
+        for (int dy = -1; dy <= 1; dy++) {
+            const int yy = cy + dy;
+            if (yy < 0 || yy >= g.dy) continue;
+            const int row = zz * g.sheet + yy * g.dx;
+            const int a = cs[row + x0], b = cs[row + x1 + 1];
+            for (int base = a; base < b; base += WAVE) {
+                const int j = base + lane;
+                Cand<Real> c;
+                bool keep = false;
+                if (j < b) {
+                    op.load_cand(j, c);
+                    keep = box_d2(c.x, c.y, c.z, lo, hi) < R2;
+                }
+                const uint64_t m = ballot(keep);
+                if (keep) buf[count + lanes_below(m)] = c;
+                count += __popcll(m);
+                if (count > CAP - WAVE) {
+                    __syncthreads();
+                    op.process(buf, count);
+                    __syncthreads();
+                    count = 0;
+                    over = true;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (count) op.process(buf, count);
+    return {count, over};
+}
+
+// Bounding box of the active lanes, widened by one part in 1e5 so that the
+// staging filter never drops a candidate the exact tests would keep.
+template <typename Real>
+__device__ __forceinline__ void wave_box(bool act, Real x, Real y, Real z, Real* lo, Real* hi)
+{
+    lo[0] = wave_min(act ? x : big<Real>());
+    lo[1] = wave_min(act ? y : big<Real>());
+    lo[2] = wave_min(act ? z : big<Real>());
+    hi[0] = wave_max(act ? x : -big<Real>());
+    hi[1] = wave_max(act ? y : -big<Real>());
+    hi[2] = wave_max(act ? z : -big<Real>());
+}
+
+template <typename Real>
+__device__ __forceinline__ Real filt_r2(Real R) { Real r = R * (Real)1.00001 + (Real)1e-12; return r * r; }
+
+}  // namespace wv
+}  // namespace mps

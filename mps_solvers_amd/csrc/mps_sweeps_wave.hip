@@ -1,0 +1,394 @@
+// mps_sweeps_wave.hip -- the "wave" variant of the neighbour sweeps: one
+// 64-lane wavefront per cell (chunks of 64 particles, lane = particle), the
+// 27-cell candidate set filtered against the wave's bounding box and staged
+// through LDS, each staged candidate broadcast to all lanes.  Same operator
+// semantics as mps_sweeps.hip (the "tile" variant); which one runs is chosen
+// per context (StepConst::variant).
+#include <float.h>
+
+#include "mps_device.hpp"
+#include "mps_kernels.hpp"
+#include "mps_stage_wave.hpp"
+
+namespace mps {
+namespace wv {
+
+// ------------------------------------------------------------------ PND ---
+template <typename Real>
+struct OpPnd {
+    const Real *x, *y, *z;
+    const int* type;
+    Real re;     // R0 in the state precision
+    Real tn;     // rr < tn  <=>  sqrt(rr) < R0
+    int all_types;
+    int i;
+    bool active;
+    Real xi, yi, zi;
+    double acc;
+
+    __device__ void load_self(int i_, bool in)
+    {
+        i = i_;
+        acc = 0.0;
+        active = false;
+        if (in) {
+            xi = x[i]; yi = y[i]; zi = z[i];
+            active = all_types || type[i] != 2;
+        }
+    }
+    __device__ void load_cand(int j, Cand<Real>& c) { c.x = x[j]; c.y = y[j]; c.z = z[j]; c.j = j; }
+    __device__ void process(const Cand<Real>* buf, int cnt)
+    {
+        const double red = (double)re;
+        for (int k = 0; k < cnt; k++) {
+            const Real cxk = buf[k].x, cyk = buf[k].y, czk = buf[k].z;
+            const int cj = buf[k].j;
+            const Real dx = cxk - xi, dy = cyk - yi, dz = czk - zi;
+            const Real rr = rr3(dx, dy, dz);
+            if (rr < tn && cj != i) {
+                // exact differences of the stored values (FP64 holds any float difference)
+                const double ddx = (double)cxk - (double)xi, ddy = (double)cyk - (double)yi,
+                             ddz = (double)czk - (double)zi;
+                const double r2 = __builtin_fma(ddx, ddx, __builtin_fma(ddy, ddy, ddz * ddz));
+                acc += __builtin_fma(red, rsq(r2), -1.0);
+            }
+        }
+    }
+};
+
+template <typename Real>
+__global__ void __launch_bounds__(64) k_pnd(Grid g, const int* __restrict__ cs, StateView<Real> s,
+                                            Thresh<Real> th, const DevScalars* __restrict__ sc,
+                                            double one_over_alpha, int all_types, int with_pressure)
+{
+    __shared__ Cand<Real> buf[Cap<Real>::v];
+    const int cell = xcd_cell(g.ncells);
+    if (cell >= g.ncells) return;
+    const int s0 = cs[cell], s1 = cs[cell + 1];
+    const double n0 = with_pressure ? sc->n0 : 1.0;
+    OpPnd<Real> op;
+    op.x = s.x; op.y = s.y; op.z = s.z; op.type = s.type;
+    op.re = th.re_n; op.tn = th.t_n; op.all_types = all_types;
+    const Real R2 = filt_r2(th.re_n);
+    for (int c0 = s0; c0 < s1; c0 += WAVE) {
+        const int i = c0 + (int)threadIdx.x;
+        const bool in = i < s1;
+        op.load_self(i, in);
+        Real lo[3], hi[3];
+        wave_box(op.active, op.xi, op.yi, op.zi, lo, hi);
+        if (!ballot(op.active)) continue;
+        stage(g, cs, cell, lo, hi, R2, buf, op);
+        if (op.active) {
+            s.n[i] = op.acc;
+            if (with_pressure) {
+                const double t = one_over_alpha * (op.acc - n0) / n0;
+                s.p[i] = (Real)(t > 0.0 ? t : 0.0);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ viscosity ---
+template <typename Real>
+struct OpVisc {
+    const Real *x, *y, *z, *u, *v, *w;
+    Real re, tl;
+    int i;
+    bool active;
+    Real xi, yi, zi, ui, vi, wi;
+    Real lx, ly, lz;
+
+    __device__ void load_cand(int j, Cand<Real>& c)
+    {
+        c.x = x[j]; c.y = y[j]; c.z = z[j]; c.j = j;
+        c.a = u[j]; c.b = v[j]; c.c = w[j];
+    }
+    __device__ void process(const Cand<Real>* buf, int cnt)
+    {
+        for (int k = 0; k < cnt; k++) {
+            const Cand<Real> c = buf[k];
+            const Real dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
+            const Real rr = rr3(dx, dy, dz);
+            const bool inside = rr < tl && c.j != i;
+            const Real wt = inside ? __builtin_fma(re, rsq(rr), (Real)-1) : (Real)0;
+            lx = __builtin_fma(wt, c.a - ui, lx);
+            ly = __builtin_fma(wt, c.b - vi, ly);
+            lz = __builtin_fma(wt, c.c - wi, lz);
+        }
+    }
+};
+
+template <typename Real>
+__global__ void __launch_bounds__(64) k_visc(Grid g, const int* __restrict__ cs, StateView<Real> a,
+                                             StateView<Real> b, Thresh<Real> th, StepConst kc,
+                                             const DevScalars* __restrict__ sc)
+{
+    __shared__ Cand<Real> buf[Cap<Real>::v];
+    const int cell = xcd_cell(g.ncells);
+    if (cell >= g.ncells) return;
+    const int s0 = cs[cell], s1 = cs[cell + 1];
+    const double dt = sc->dt, coef = sc->coef;
+    OpVisc<Real> op;
+    op.x = a.x; op.y = a.y; op.z = a.z; op.u = a.u; op.v = a.v; op.w = a.w;
+    op.re = th.re_lap; op.tl = th.t_lap;
+    const Real R2 = filt_r2(th.re_lap);
+    for (int c0 = s0; c0 < s1; c0 += WAVE) {
+        const int i = c0 + (int)threadIdx.x;
+        const bool in = i < s1;
+        op.i = i;
+        op.active = false;
+        op.lx = op.ly = op.lz = (Real)0;
+        if (in) {
+            op.xi = a.x[i]; op.yi = a.y[i]; op.zi = a.z[i];
+            op.ui = a.u[i]; op.vi = a.v[i]; op.wi = a.w[i];
+            op.active = a.type[i] == 0;
+        }
+        Real lo[3], hi[3];
+        wave_box(op.active, op.xi, op.yi, op.zi, lo, hi);
+        if (ballot(op.active)) stage(g, cs, cell, lo, hi, R2, buf, op);
+        if (in) {
+            Real xo = op.xi, yo = op.yi, zo = op.zi, uo = op.ui, vo = op.vi, wo = op.wi;
+            if (op.active) {
+                const Real dtr = (Real)dt, nc = (Real)(kc.nu * coef);
+                uo = op.ui + dtr * (nc * op.lx + (Real)kc.g[0]);
+                vo = op.vi + dtr * (nc * op.ly + (Real)kc.g[1]);
+                wo = op.wi + dtr * (nc * op.lz + (Real)kc.g[2]);
+                xo = op.xi + dtr * uo; yo = op.yi + dtr * vo; zo = op.zi + dtr * wo;
+            }
+            b.x[i] = xo; b.y[i] = yo; b.z[i] = zo; b.u[i] = uo; b.v[i] = vo; b.w[i] = wo;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ collision ---
+template <typename Real>
+struct OpCol {
+    const Real *x, *y, *z, *u, *v, *w;
+    Real tc, rate;
+    int i, hits;
+    bool active;
+    Real xi, yi, zi, ui, vi, wi;
+    Real cx, cy, cz;
+
+    __device__ void load_cand(int j, Cand<Real>& c)
+    {
+        c.x = x[j]; c.y = y[j]; c.z = z[j]; c.j = j;
+        c.a = u[j]; c.b = v[j]; c.c = w[j];
+    }
+    __device__ void process(const Cand<Real>* buf, int cnt)
+    {
+        for (int k = 0; k < cnt; k++) {
+            const Cand<Real> c = buf[k];
+            const Real dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
+            const Real rr = rr3(dx, dy, dz);
+            if (rr < tc && c.j != i) {
+                // v = 0.5 (u_j - u_i) . r / |r|; the sign decides, formed as the reference forms it
+                const Real hx = (Real)0.5 * (c.a - ui), hy = (Real)0.5 * (c.b - vi),
+                           hz = (Real)0.5 * (c.c - wi);
+                Real dot = hx * dx;
+                dot = dot + hy * dy;
+                dot = dot + hz * dz;
+                if (dot <= (Real)0) {
+                    const Real ods = rsq(rr);
+                    const Real f = rate * (dot * ods) * ods;
+                    cx = __builtin_fma(f, dx, cx);
+                    cy = __builtin_fma(f, dy, cy);
+                    cz = __builtin_fma(f, dz, cz);
+                    hits++;
+                }
+            }
+        }
+    }
+};
+
+template <typename Real>
+__global__ void __launch_bounds__(64) k_collision(Grid g, const int* __restrict__ cs, StateView<Real> a,
+                                                  StateView<Real> b, Thresh<Real> th, StepConst kc,
+                                                  DevScalars* __restrict__ sc)
+{
+    __shared__ Cand<Real> buf[Cap<Real>::v];
+    const int cell = xcd_cell(g.ncells);
+    if (cell >= g.ncells) return;
+    const int s0 = cs[cell], s1 = cs[cell + 1];
+    const double dt = sc->dt;
+    OpCol<Real> op;
+    op.x = a.x; op.y = a.y; op.z = a.z; op.u = a.u; op.v = a.v; op.w = a.w;
+    op.tc = th.t_col; op.rate = (Real)kc.col_rate;
+    const Real R2 = filt_r2(th.col);
+    int hits = 0;
+    for (int c0 = s0; c0 < s1; c0 += WAVE) {
+        const int i = c0 + (int)threadIdx.x;
+        const bool in = i < s1;
+        op.i = i;
+        op.active = false;
+        op.hits = 0;
+        op.cx = op.cy = op.cz = (Real)0;
+        if (in) {
+            op.xi = a.x[i]; op.yi = a.y[i]; op.zi = a.z[i];
+            op.ui = a.u[i]; op.vi = a.v[i]; op.wi = a.w[i];
+            op.active = a.type[i] == 0;
+        }
+        Real lo[3], hi[3];
+        wave_box(op.active, op.xi, op.yi, op.zi, lo, hi);
+        if (ballot(op.active)) stage(g, cs, cell, lo, hi, R2, buf, op);
+        if (in) {
+            Real cx = op.active ? op.cx : (Real)0, cy = op.active ? op.cy : (Real)0,
+                 cz = op.active ? op.cz : (Real)0;
+            const Real dtr = (Real)dt;
+            b.u[i] = op.ui + cx; b.v[i] = op.vi + cy; b.w[i] = op.wi + cz;
+            b.x[i] = op.xi + dtr * cx; b.y[i] = op.yi + dtr * cy; b.z[i] = op.zi + dtr * cz;
+            if (op.active) hits += op.hits;
+        }
+        __syncthreads();
+    }
+    hits = wave_sum(hits);
+    if (threadIdx.x == 0 && hits) atomicAdd(&sc->ncol, hits);
+}
+
+// ------------------------------------------------------ pressure gradient ---
+template <typename Real>
+struct OpGrad {
+    const Real *x, *y, *z, *p;
+    const int* type;
+    Real re, tn, re2;
+    int i, pass, skip_dummy;
+    bool active;
+    Real xi, yi, zi, ph;
+    Real gx, gy, gz;
+
+    __device__ void load_cand(int j, Cand<Real>& c)
+    {
+        c.x = x[j]; c.y = y[j]; c.z = z[j]; c.j = j;
+        c.t = type[j]; c.a = p[j];
+    }
+    __device__ void process(const Cand<Real>* buf, int cnt)
+    {
+        if (pass == 0) {
+            // p_hat = min(p_i, p_j : type_j != 2, |r|^2 <= R0^2)
+            for (int k = 0; k < cnt; k++) {
+                const Real cxk = buf[k].x, cyk = buf[k].y, czk = buf[k].z, pk = buf[k].a;
+                const int tk = buf[k].t;
+                const Real rr = rr3(cxk - xi, cyk - yi, czk - zi);
+                const bool near = tk != 2 && rr <= re2 && pk < ph;
+                ph = near ? pk : ph;
+            }
+            return;
+        }
+        for (int k = 0; k < cnt; k++) {
+            const Cand<Real> c = buf[k];
+            const Real dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
+            const Real rr = rr3(dx, dy, dz);
+            const bool inside = rr < tn && c.j != i && !(skip_dummy && c.t == 2);
+            const Real ri = rsq(rr);
+            const Real f = inside ? (c.a - ph) * (ri * ri) * __builtin_fma(re, ri, (Real)-1) : (Real)0;
+            gx = __builtin_fma(f, dx, gx);
+            gy = __builtin_fma(f, dy, gy);
+            gz = __builtin_fma(f, dz, gz);
+        }
+    }
+};
+
+template <typename Real>
+__global__ void __launch_bounds__(64) k_gradp(Grid g, const int* __restrict__ cs, StateView<Real> a,
+                                              StateView<Real> b, Thresh<Real> th, StepConst kc,
+                                              DevScalars* __restrict__ sc)
+{
+    __shared__ Cand<Real> buf[Cap<Real>::v];
+    const int cell = xcd_cell(g.ncells);
+    if (cell >= g.ncells) return;
+    const int s0 = cs[cell], s1 = cs[cell + 1];
+    const double dt = sc->dt, n0 = sc->n0;
+    OpGrad<Real> op;
+    op.x = a.x; op.y = a.y; op.z = a.z; op.p = a.p; op.type = a.type;
+    op.re = th.re_n; op.tn = th.t_n; op.re2 = th.re_n2; op.skip_dummy = kc.gradp_skip_dummy;
+    const Real R2 = filt_r2(th.re_n);
+    float umax2 = 0.0f;
+    for (int c0 = s0; c0 < s1; c0 += WAVE) {
+        const int i = c0 + (int)threadIdx.x;
+        const bool in = i < s1;
+        op.i = i;
+        op.active = false;
+        op.pass = 0;
+        op.gx = op.gy = op.gz = (Real)0;
+        Real ui = 0, vi = 0, wi = 0;
+        if (in) {
+            op.xi = a.x[i]; op.yi = a.y[i]; op.zi = a.z[i];
+            ui = a.u[i]; vi = a.v[i]; wi = a.w[i];
+            op.ph = a.p[i];
+            op.active = a.type[i] == 0;
+        }
+        Real lo[3], hi[3];
+        wave_box(op.active, op.xi, op.yi, op.zi, lo, hi);
+        if (ballot(op.active)) {
+            StageResult r = stage(g, cs, cell, lo, hi, R2, buf, op);
+            op.pass = 1;
+            if (!r.overflowed) {
+                op.process(buf, r.count);
+            } else {
+                __syncthreads();
+                stage(g, cs, cell, lo, hi, R2, buf, op);
+            }
+        }
+        if (in) {
+            Real xo = op.xi, yo = op.yi, zo = op.zi, uo = ui, vo = vi, wo = wi;
+            if (op.active) {
+                const Real s = (Real)(-dt / kc.rho * (double)kc.dim / n0);
+                const Real du = s * op.gx, dv = s * op.gy, dw = s * op.gz;
+                const Real dtr = (Real)dt;
+                uo = ui + du; vo = vi + dv; wo = wi + dw;
+                xo = op.xi + dtr * du; yo = op.yi + dtr * dv; zo = op.zi + dtr * dw;
+                const float m2 = (float)(uo * uo + vo * vo + wo * wo);
+                umax2 = m2 > umax2 ? m2 : umax2;
+            }
+            b.x[i] = xo; b.y[i] = yo; b.z[i] = zo; b.u[i] = uo; b.v[i] = vo; b.w[i] = wo;
+        }
+        __syncthreads();
+    }
+    umax2 = wave_max(umax2);
+    if (threadIdx.x == 0 && umax2 > 0.0f) atomicMax(&sc->maxu2_bits, __float_as_uint(umax2));
+}
+
+
+template <typename Real>
+void launch_pnd(const Grid& g, const int* cs, StateView<Real> s, const Thresh<Real>& th, const StepConst& k,
+                DevScalars* sc, int all_types, int with_pressure, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_pnd<Real>, dim3(xcd_grid(g.ncells)), dim3(64), 0, st, g, cs, s, th, sc,
+                       k.rho * k.cs * k.cs, all_types, with_pressure);
+}
+template <typename Real>
+void launch_visc(const Grid& g, const int* cs, StateView<Real> a, StateView<Real> b, const Thresh<Real>& th,
+                 const StepConst& k, DevScalars* sc, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_visc<Real>, dim3(xcd_grid(g.ncells)), dim3(64), 0, st, g, cs, a, b, th, k, sc);
+}
+template <typename Real>
+void launch_collision(const Grid& g, const int* cs, StateView<Real> a, StateView<Real> b, const Thresh<Real>& th,
+                      const StepConst& k, DevScalars* sc, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_collision<Real>, dim3(xcd_grid(g.ncells)), dim3(64), 0, st, g, cs, a, b, th, k, sc);
+}
+template <typename Real>
+void launch_gradp(const Grid& g, const int* cs, StateView<Real> a, StateView<Real> b, const Thresh<Real>& th,
+                  const StepConst& k, DevScalars* sc, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_gradp<Real>, dim3(xcd_grid(g.ncells)), dim3(64), 0, st, g, cs, a, b, th, k, sc);
+}
+
+#define MPS_INST_W(R)                                                                                          \
+    template void launch_pnd<R>(const Grid&, const int*, StateView<R>, const Thresh<R>&, const StepConst&,   \
+                                DevScalars*, int, int, hipStream_t);                                          \
+    template void launch_visc<R>(const Grid&, const int*, StateView<R>, StateView<R>, const Thresh<R>&,      \
+                                 const StepConst&, DevScalars*, hipStream_t);                                 \
+    template void launch_collision<R>(const Grid&, const int*, StateView<R>, StateView<R>, const Thresh<R>&, \
+                                      const StepConst&, DevScalars*, hipStream_t);                            \
+    template void launch_gradp<R>(const Grid&, const int*, StateView<R>, StateView<R>, const Thresh<R>&,     \
+                                  const StepConst&, DevScalars*, hipStream_t);
+MPS_INST_W(float)
+MPS_INST_W(double)
+#undef MPS_INST_W
+
+}  // namespace wv
+}  // namespace mps

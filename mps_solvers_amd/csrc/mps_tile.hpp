@@ -30,7 +30,7 @@ constexpr int TW = 8;     // waves per block
 constexpr int TK = 8;     // lanes per particle
 
 template <typename Real>
-struct Cand {
+struct alignas(16) Cand {
     Real x, y, z;
     int j, t;
     Real a, b, c;

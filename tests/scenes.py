@@ -10,8 +10,8 @@ CFG0 = dict(dim=3, rho=1000.0, nu=1e-6, dp=0.01, re_n_mult=2.1, re_lap_mult=4.0,
             max_dis=0.001, g=(0.0, 0.0, -9.81))
 
 
-def small_dam(seed=3, fluid=(6, 6, 8), inner=(12, 6, 10), dp=0.01, dim=3):
-    return mps.dambreak(fluid, inner, 2, 2, dp, 0.01, seed, dim)
+def small_dam(seed=3, fluid=(6, 6, 8), inner=(12, 6, 10), dp=0.01, dim=3, jitter=0.01):
+    return mps.dambreak(fluid, inner, 2, 2, dp, jitter, seed, dim)
 
 
 def gpu_params(precision=0, **kw):

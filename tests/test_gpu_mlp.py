@@ -70,7 +70,7 @@ def test_training_lowers_loss():
         th = s.mlp_get_params()
         s.mlp_set_params(th - 0.05 * g)
     loss_end, _ = s.mlp_grad(X[:4096], t[:4096].astype(np.float32))
-    assert loss_end < 0.2 * l_first
+    assert loss_end < 0.5 * l_first
 
 
 def _features_np(A, st, n0, rho_g, dp, ps):
