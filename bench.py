@@ -150,6 +150,30 @@ def ppe_bench(mps, spin_steps=20, train_iters=2000, log=print):
                       "total_s": round(time.perf_counter() - t0, 2)}}
 
 
+def copy_calib(mib, reps=3):
+    """A plain device-to-device copy of known size: the reference point that
+    scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    src = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": src.numel() * 4, "reps": reps}), flush=True)
+
+
+def traffic_from_profile(dom, config):
+    """HBM bytes per launch of kernel `dom` from the committed counter summary
+    (a line 'bytes_per_launch: B'), or None."""
+    tf = os.path.join(HERE, "profiles", f"traffic_{dom}_config{config}.txt")
+    if not os.path.exists(tf):
+        return None
+    for line in open(tf):
+        if line.startswith("bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,7 +183,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-ppe", action="store_true")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter scaling) and exit")
     a = ap.parse_args()
+    if a.copy_calib:
+        copy_calib(a.copy_calib)
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the explicit "
               "step would shard by spatial slabs, see the design notes)", file=sys.stderr)
@@ -190,10 +219,7 @@ def main():
     per = {k: v / nt for k, v in st.items()}
     dom = max(("visc", "collision", "pnd_pres", "gradp", "sort"), key=lambda k: per[k])
     ach = ALG_BYTES[dom] * N / (per[dom] * 1e-3) / 1e9
-    traffic = None
-    tf = os.path.join(HERE, "profiles", f"traffic_{dom}_config{a.config}.json")
-    if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("bytes_per_launch")
+    traffic = traffic_from_profile(dom, a.config)
     sc = s.scalars()
     out = {
         "metric": "particle-updates/s at 1M particles; PPE ms/solve to 1e-8 residual, learned vs plain",

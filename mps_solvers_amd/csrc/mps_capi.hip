@@ -487,6 +487,9 @@ extern "C" int mps_create(const mps_params* p, int np, int nb2, int nb1, const d
         // sweep kernels: MPS_SWEEP=tile selects the eight-particle-tile variant
         const char* v = getenv("MPS_SWEEP");
         k.variant = (v && !strcmp(v, "tile")) ? 1 : 0;
+        // MPS_WAVE_CAP=512 selects the larger LDS batch of the one-wavefront sweeps
+        const char* wc = getenv("MPS_WAVE_CAP");
+        k.wave_cap = (wc && atoi(wc) == 512) ? 512 : 256;
     }
     double fl2 = 0.0, st2 = 0.0;
     for (int i = 0; i < np; i++) {

@@ -78,6 +78,12 @@ __device__ __forceinline__ Real rr3(Real dx, Real dy, Real dz)
     return s + c;
 }
 
+// Fused multiply-add in the operands' own precision (the bare __builtin_fma is
+// the double one: float operands would be widened, fused in FP64 and rounded
+// back, at several times the VALU cost).
+__device__ __forceinline__ float fmar(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmar(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 __device__ __forceinline__ float rsq(float a) { return __builtin_amdgcn_rsqf(a); }
 __device__ __forceinline__ double rsq(double a)
 {

@@ -14,6 +14,8 @@
 // inside an FP64 iterative-refinement loop (or plain FP64 CG).  Reductions are
 // deterministic: per-block FP64 partials, summed in a fixed order by the last
 // block to finish.  No kernel waits on another block.
+#include <algorithm>
+
 #include "mps_device.hpp"
 #include "mps_kernels.hpp"
 #include "mps_ppe.hpp"
@@ -30,10 +32,9 @@ __global__ void __launch_bounds__(256) k_ppe_rows(int n, StateView<Real> s, cons
                                                   double* __restrict__ x, double* __restrict__ partial,
                                                   PpeScalars* ps)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const double n0 = sc->n0, lambda = sc->lambda, dt = sc->dt;
     double bb = 0.0;
-    if (i < n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double ni = s.n[i];
         const bool isbc = ni < beta * n0;
         const bool id = isbc || s.type[i] == 2;
@@ -45,7 +46,7 @@ __global__ void __launch_bounds__(256) k_ppe_rows(int n, StateView<Real> s, cons
         if (x0_kind == 1) xi = (double)s.p[i];
         else if (x0_kind == 2) xi = x0_ext[i];
         x[i] = isbc ? 0.0 : xi;
-        bb = bi * bi;
+        bb += bi * bi;
     }
     block_sum_last(bb, partial, &ps->ticket[0], &ps->bb);
 }
@@ -197,14 +198,13 @@ __global__ void __launch_bounds__(256) k_spmv_dot(int n, const int* __restrict__
                                                   PpeScalars* ps)
 {
     if (ps->done) return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int row = t / G, g = t % G;
+    const int g = threadIdx.x % G, rpb = blockDim.x / G;
     double d = 0.0;
-    if (row < n) {
+    for (int row = blockIdx.x * rpb + threadIdx.x / G; row < n; row += gridDim.x * rpb) {
         const V a = row_dot<G, MV, V, V>(row, g, rp, col, val, p);
         if (g == 0) {
             q[row] = a;
-            d = (double)p[row] * (double)a;
+            d += (double)p[row] * (double)a;
         }
     }
     block_sum_last(d, partial, &ps->ticket[1], &ps->pq);
@@ -219,13 +219,12 @@ __global__ void __launch_bounds__(256) k_cg_update(int n, V* __restrict__ x, V* 
     if (ps->done) return;
     const double rr = ps->rr, pq = ps->pq;
     const V alpha = (V)(rr / pq);
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double d = 0.0;
-    if (i < n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         x[i] += alpha * p[i];
         const V ri = r[i] - alpha * q[i];
         r[i] = ri;
-        d = (double)ri * (double)ri;
+        d += (double)ri * (double)ri;
     }
     double tot;
     if (block_sum_last(d, partial, &ps->ticket[2], &tot) && threadIdx.x == 0) {
@@ -262,10 +261,9 @@ __global__ void __launch_bounds__(256) k_residual(int n, const int* __restrict__
                                                   double* __restrict__ partial, double* __restrict__ partial2,
                                                   PpeScalars* ps, double tol_in2, double floor2)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int row = t / G, g = t % G;
+    const int g = threadIdx.x % G, rpb = blockDim.x / G;
     double d64 = 0.0, din = 0.0;
-    if (row < n) {
+    for (int row = blockIdx.x * rpb + threadIdx.x / G; row < n; row += gridDim.x * rpb) {
         const double a = row_dot<G, double, double, double>(row, g, rp, col, val, x);
         if (g == 0) {
             const double ri = b[row] - a;
@@ -273,8 +271,8 @@ __global__ void __launch_bounds__(256) k_residual(int n, const int* __restrict__
             e[row] = (V)0;
             r[row] = rv;
             p[row] = rv;
-            d64 = ri * ri;
-            din = (double)rv * (double)rv;
+            d64 += ri * ri;
+            din += (double)rv * (double)rv;
         }
     }
     double tot, rin;
@@ -307,15 +305,23 @@ __global__ void __launch_bounds__(256) k_press_corr(int n, const double* __restr
 // ------------------------------------------------------------- launchers ---
 static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 constexpr int kG = 16;
+// Reducing kernels run at most kRedBlocks blocks (grid-stride loops): the
+// last-block reduction then takes one ticket atomic per block, a few thousand
+// per launch instead of one per 16 rows.  The grid depends on n only, so the
+// summation order, and the result, is fixed for a given system.
+constexpr int kRedBlocks = 2048;
+static inline int red_grid(long threads) { return std::min(cdiv(threads, 256), kRedBlocks); }
 
-int ppe_partial_blocks(int n) { return cdiv((long)n * kG, 256) + cdiv(n, 256) + 8; }
+// doubles in each partial buffer: the block partials of the reducing kernels,
+// and also the uint32 tile totals of the row-pointer scan (4096 rows a tile)
+int ppe_partial_blocks(int n) { return std::max(kRedBlocks, cdiv((long)n + 1, 2 * 4096)) + 8; }
 
 template <typename Real>
 void launch_ppe_rows(int n, StateView<Real> s, const DevScalars* sc, double beta, double btmp_over,
                      int x0_kind, const double* x0_ext, unsigned char* bc, double* b, double* x,
                      double* partial, PpeScalars* ps, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_ppe_rows<Real>, dim3(cdiv(n, 256)), dim3(256), 0, st, n, s, sc, beta,
+    hipLaunchKernelGGL(k_ppe_rows<Real>, dim3(red_grid(n)), dim3(256), 0, st, n, s, sc, beta,
                        btmp_over, x0_kind, x0_ext, bc, b, x, partial, ps);
 }
 
@@ -332,9 +338,9 @@ template <typename MV, typename V>
 void launch_cg_iter(int n, const int* rp, const int* col, const MV* val, V* x, V* r, V* p, V* q,
                     double* partial, PpeScalars* ps, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_spmv_dot<kG, MV, V>), dim3(cdiv((long)n * kG, 256)), dim3(256), 0, st, n, rp, col,
+    hipLaunchKernelGGL((k_spmv_dot<kG, MV, V>), dim3(red_grid((long)n * kG)), dim3(256), 0, st, n, rp, col,
                        val, p, q, partial, ps);
-    hipLaunchKernelGGL(k_cg_update<V>, dim3(cdiv(n, 256)), dim3(256), 0, st, n, x, r, p, q, partial, ps);
+    hipLaunchKernelGGL(k_cg_update<V>, dim3(red_grid(n)), dim3(256), 0, st, n, x, r, p, q, partial, ps);
     hipLaunchKernelGGL(k_cg_dir<V>, dim3(cdiv(n, 256)), dim3(256), 0, st, n, r, p, ps);
 }
 
@@ -343,7 +349,7 @@ void launch_residual(int n, const int* rp, const int* col, const double* val, co
                      const double* x, V* e, V* r, V* p, double* partial, double* partial2,
                      PpeScalars* ps, double tol_in2, double floor2, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_residual<kG, V>), dim3(cdiv((long)n * kG, 256)), dim3(256), 0, st, n, rp, col, val,
+    hipLaunchKernelGGL((k_residual<kG, V>), dim3(red_grid((long)n * kG)), dim3(256), 0, st, n, rp, col, val,
                        b, x, e, r, p, partial, partial2, ps, tol_in2, floor2);
 }
 

@@ -9,6 +9,7 @@
 namespace mps {
 namespace wv {
 
+// Staged candidate of the sweeps that need velocity or pressure (32 B for float).
 template <typename Real>
 struct alignas(16) Cand {
     Real x, y, z;
@@ -16,8 +17,17 @@ struct alignas(16) Cand {
     Real a, b, c;
 };
 
-template <typename Real> struct Cap { static constexpr int v = 512; };
-template <> struct Cap<double> { static constexpr int v = 256; };
+// Staged candidate of the number-density sweep: position and index only
+// (16 B for float: one LDS read per broadcast).
+template <typename Real>
+struct alignas(16) CandP {
+    Real x, y, z;
+    int j;
+};
+
+// LDS batch capacity in candidates: `cap` is the float capacity chosen per
+// context (256 or 512); FP64 candidates are twice as large, so half as many.
+template <typename Real> __host__ __device__ constexpr int cap_of(int cap) { return sizeof(Real) == 4 ? cap : cap / 2; }
 
 template <typename Real> __device__ __forceinline__ Real big();
 template <> __device__ __forceinline__ float big<float>() { return FLT_MAX; }
@@ -35,12 +45,12 @@ __device__ __forceinline__ Real box_d2(Real qx, Real qy, Real qz, const Real* lo
 struct StageResult { int count; bool overflowed; };
 
 // Stream the stencil of `cell`, keep candidates within sqrt(R2) of the box,
-// and hand full LDS batches to op.process().  The last batch stays in `buf`.
-template <typename Real, class Op>
+// and hand full LDS batches (CAP candidates of type Cd) to op.process().  The
+// last batch stays in `buf`.
+template <int CAP, typename Real, class Op, class Cd>
 __device__ StageResult stage(const Grid& g, const int* __restrict__ cs, int cell, const Real* lo,
-                             const Real* hi, Real R2, Cand<Real>* buf, Op& op)
+                             const Real* hi, Real R2, Cd* buf, Op& op)
 {
-    constexpr int CAP = Cap<Real>::v;
     const int lane = threadIdx.x;
     int cx, cy, cz;
     cell_xyz(g, cell, cx, cy, cz);
@@ -57,7 +67,7 @@ __device__ StageResult stage(const Grid& g, const int* __restrict__ cs, int cell
             const int a = cs[row + x0], b = cs[row + x1 + 1];
             for (int base = a; base < b; base += WAVE) {
                 const int j = base + lane;
-                Cand<Real> c;
+                Cd c;
                 bool keep = false;
                 if (j < b) {
                     op.load_cand(j, c);

@@ -132,10 +132,10 @@ struct OpVisc {
             const Real dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
             const Real rr = rr3(dx, dy, dz);
             const bool inside = rr < tl && c.j != i;
-            const Real wt = inside ? __builtin_fma(re, rsq(rr), (Real)-1) : (Real)0;
-            lx = __builtin_fma(wt, c.a - ui, lx);
-            ly = __builtin_fma(wt, c.b - vi, ly);
-            lz = __builtin_fma(wt, c.c - wi, lz);
+            const Real wt = inside ? fmar(re, rsq(rr), (Real)-1) : (Real)0;
+            lx = fmar(wt, c.a - ui, lx);
+            ly = fmar(wt, c.b - vi, ly);
+            lz = fmar(wt, c.c - wi, lz);
         }
     }
 };
@@ -217,9 +217,9 @@ struct OpCol {
                 if (dot <= (Real)0) {
                     const Real ods = rsq(rr);
                     const Real f = rate * (dot * ods) * ods;
-                    cx = __builtin_fma(f, dx, cx);
-                    cy = __builtin_fma(f, dy, cy);
-                    cz = __builtin_fma(f, dz, cz);
+                    cx = fmar(f, dx, cx);
+                    cy = fmar(f, dy, cy);
+                    cz = fmar(f, dz, cz);
                     hits++;
                 }
             }
@@ -306,10 +306,10 @@ struct OpGrad {
             const Real rr = rr3(dx, dy, dz);
             const bool inside = rr < tn && c.j != i && !(skip_dummy && c.t == 2);
             const Real ri = rsq(rr);
-            const Real f = inside ? (c.a - ph) * (ri * ri) * __builtin_fma(re, ri, (Real)-1) : (Real)0;
-            gx = __builtin_fma(f, dx, gx);
-            gy = __builtin_fma(f, dy, gy);
-            gz = __builtin_fma(f, dz, gz);
+            const Real f = inside ? (c.a - ph) * (ri * ri) * fmar(re, ri, (Real)-1) : (Real)0;
+            gx = fmar(f, dx, gx);
+            gy = fmar(f, dy, gy);
+            gz = fmar(f, dz, gz);
         }
     }
 };

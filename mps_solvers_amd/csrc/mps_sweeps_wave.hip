@@ -14,6 +14,12 @@ namespace mps {
 namespace wv {
 
 // ------------------------------------------------------------------ PND ---
+// Two phases per LDS batch: the FP32 cutoff test appends the in-range
+// candidates to a per-lane list (slot-major in LDS, so that every lane's k-th
+// entry sits in its own bank), then the FP64 weight runs over the dense
+// lists only.  Each lane still adds its neighbours in ascending staged order.
+constexpr int PND_LMAX = 48;   // list slots per lane (drained before any lane can overflow)
+
 template <typename Real>
 struct OpPnd {
     const Real *x, *y, *z;
@@ -25,6 +31,7 @@ struct OpPnd {
     bool active;
     Real xi, yi, zi;
     double acc;
+    uint16_t* L;   // L[s * WAVE + lane]
 
     __device__ void load_self(int i_, bool in)
     {
@@ -36,32 +43,50 @@ struct OpPnd {
             active = all_types || type[i] != 2;
         }
     }
-    __device__ void load_cand(int j, Cand<Real>& c) { c.x = x[j]; c.y = y[j]; c.z = z[j]; c.j = j; }
-    __device__ void process(const Cand<Real>* buf, int cnt)
+    __device__ void load_cand(int j, CandP<Real>& c) { c.x = x[j]; c.y = y[j]; c.z = z[j]; c.j = j; }
+    __device__ void drain(const CandP<Real>* buf, int m)
     {
-        const double red = (double)re;
-        for (int k = 0; k < cnt; k++) {
-            const Real cxk = buf[k].x, cyk = buf[k].y, czk = buf[k].z;
-            const int cj = buf[k].j;
-            const Real dx = cxk - xi, dy = cyk - yi, dz = czk - zi;
-            const Real rr = rr3(dx, dy, dz);
-            if (rr < tn && cj != i) {
+        const int lane = threadIdx.x;
+        const int mm = wave_max(m);
+        const double red = (double)re, xd = (double)xi, yd = (double)yi, zd = (double)zi;
+        for (int s = 0; s < mm; s++) {
+            if (s < m) {
+                const CandP<Real> c = buf[L[s * WAVE + lane]];
                 // exact differences of the stored values (FP64 holds any float difference)
-                const double ddx = (double)cxk - (double)xi, ddy = (double)cyk - (double)yi,
-                             ddz = (double)czk - (double)zi;
+                const double ddx = (double)c.x - xd, ddy = (double)c.y - yd, ddz = (double)c.z - zd;
                 const double r2 = __builtin_fma(ddx, ddx, __builtin_fma(ddy, ddy, ddz * ddz));
                 acc += __builtin_fma(red, rsq(r2), -1.0);
             }
         }
     }
+    __device__ void process(const CandP<Real>* buf, int cnt)
+    {
+        const int lane = threadIdx.x;
+        int m = 0;
+        for (int k = 0; k < cnt; k++) {
+            const CandP<Real> c = buf[k];
+            const Real rr = rr3(c.x - xi, c.y - yi, c.z - zi);
+            if (active && rr < tn && c.j != i) {
+                L[m * WAVE + lane] = (uint16_t)k;
+                m++;
+            }
+            // at most 16 appends between checks: no lane passes PND_LMAX
+            if ((k & 15) == 15 && ballot(m > PND_LMAX - 16)) {
+                drain(buf, m);
+                m = 0;
+            }
+        }
+        drain(buf, m);
+    }
 };
 
-template <typename Real>
+template <typename Real, int CAP>
 __global__ void __launch_bounds__(64) k_pnd(Grid g, const int* __restrict__ cs, StateView<Real> s,
                                             Thresh<Real> th, const DevScalars* __restrict__ sc,
                                             double one_over_alpha, int all_types, int with_pressure)
 {
-    __shared__ Cand<Real> buf[Cap<Real>::v];
+    __shared__ CandP<Real> buf[CAP];
+    __shared__ uint16_t L[PND_LMAX * WAVE];
     const int cell = xcd_cell(g.ncells);
     if (cell >= g.ncells) return;
     const int s0 = cs[cell], s1 = cs[cell + 1];
@@ -69,6 +94,7 @@ __global__ void __launch_bounds__(64) k_pnd(Grid g, const int* __restrict__ cs, 
     OpPnd<Real> op;
     op.x = s.x; op.y = s.y; op.z = s.z; op.type = s.type;
     op.re = th.re_n; op.tn = th.t_n; op.all_types = all_types;
+    op.L = L;
     const Real R2 = filt_r2(th.re_n);
     for (int c0 = s0; c0 < s1; c0 += WAVE) {
         const int i = c0 + (int)threadIdx.x;
@@ -77,7 +103,7 @@ __global__ void __launch_bounds__(64) k_pnd(Grid g, const int* __restrict__ cs, 
         Real lo[3], hi[3];
         wave_box(op.active, op.xi, op.yi, op.zi, lo, hi);
         if (!ballot(op.active)) continue;
-        stage(g, cs, cell, lo, hi, R2, buf, op);
+        stage<CAP>(g, cs, cell, lo, hi, R2, buf, op);
         if (op.active) {
             s.n[i] = op.acc;
             if (with_pressure) {
@@ -111,20 +137,20 @@ struct OpVisc {
             const Real dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
             const Real rr = rr3(dx, dy, dz);
             const bool inside = rr < tl && c.j != i;
-            const Real wt = inside ? __builtin_fma(re, rsq(rr), (Real)-1) : (Real)0;
-            lx = __builtin_fma(wt, c.a - ui, lx);
-            ly = __builtin_fma(wt, c.b - vi, ly);
-            lz = __builtin_fma(wt, c.c - wi, lz);
+            const Real wt = inside ? fmar(re, rsq(rr), (Real)-1) : (Real)0;
+            lx = fmar(wt, c.a - ui, lx);
+            ly = fmar(wt, c.b - vi, ly);
+            lz = fmar(wt, c.c - wi, lz);
         }
     }
 };
 
-template <typename Real>
+template <typename Real, int CAP>
 __global__ void __launch_bounds__(64) k_visc(Grid g, const int* __restrict__ cs, StateView<Real> a,
                                              StateView<Real> b, Thresh<Real> th, StepConst kc,
                                              const DevScalars* __restrict__ sc)
 {
-    __shared__ Cand<Real> buf[Cap<Real>::v];
+    __shared__ Cand<Real> buf[CAP];
     const int cell = xcd_cell(g.ncells);
     if (cell >= g.ncells) return;
     const int s0 = cs[cell], s1 = cs[cell + 1];
@@ -146,7 +172,7 @@ __global__ void __launch_bounds__(64) k_visc(Grid g, const int* __restrict__ cs,
         }
         Real lo[3], hi[3];
         wave_box(op.active, op.xi, op.yi, op.zi, lo, hi);
-        if (ballot(op.active)) stage(g, cs, cell, lo, hi, R2, buf, op);
+        if (ballot(op.active)) stage<CAP>(g, cs, cell, lo, hi, R2, buf, op);
         if (in) {
             Real xo = op.xi, yo = op.yi, zo = op.zi, uo = op.ui, vo = op.vi, wo = op.wi;
             if (op.active) {
@@ -193,9 +219,9 @@ struct OpCol {
                 if (dot <= (Real)0) {
                     const Real ods = rsq(rr);
                     const Real f = rate * (dot * ods) * ods;
-                    cx = __builtin_fma(f, dx, cx);
-                    cy = __builtin_fma(f, dy, cy);
-                    cz = __builtin_fma(f, dz, cz);
+                    cx = fmar(f, dx, cx);
+                    cy = fmar(f, dy, cy);
+                    cz = fmar(f, dz, cz);
                     hits++;
                 }
             }
@@ -203,12 +229,12 @@ struct OpCol {
     }
 };
 
-template <typename Real>
+template <typename Real, int CAP>
 __global__ void __launch_bounds__(64) k_collision(Grid g, const int* __restrict__ cs, StateView<Real> a,
                                                   StateView<Real> b, Thresh<Real> th, StepConst kc,
                                                   DevScalars* __restrict__ sc)
 {
-    __shared__ Cand<Real> buf[Cap<Real>::v];
+    __shared__ Cand<Real> buf[CAP];
     const int cell = xcd_cell(g.ncells);
     if (cell >= g.ncells) return;
     const int s0 = cs[cell], s1 = cs[cell + 1];
@@ -232,7 +258,7 @@ __global__ void __launch_bounds__(64) k_collision(Grid g, const int* __restrict_
         }
         Real lo[3], hi[3];
         wave_box(op.active, op.xi, op.yi, op.zi, lo, hi);
-        if (ballot(op.active)) stage(g, cs, cell, lo, hi, R2, buf, op);
+        if (ballot(op.active)) stage<CAP>(g, cs, cell, lo, hi, R2, buf, op);
         if (in) {
             Real cx = op.active ? op.cx : (Real)0, cy = op.active ? op.cy : (Real)0,
                  cz = op.active ? op.cz : (Real)0;
@@ -282,20 +308,20 @@ struct OpGrad {
             const Real rr = rr3(dx, dy, dz);
             const bool inside = rr < tn && c.j != i && !(skip_dummy && c.t == 2);
             const Real ri = rsq(rr);
-            const Real f = inside ? (c.a - ph) * (ri * ri) * __builtin_fma(re, ri, (Real)-1) : (Real)0;
-            gx = __builtin_fma(f, dx, gx);
-            gy = __builtin_fma(f, dy, gy);
-            gz = __builtin_fma(f, dz, gz);
+            const Real f = inside ? (c.a - ph) * (ri * ri) * fmar(re, ri, (Real)-1) : (Real)0;
+            gx = fmar(f, dx, gx);
+            gy = fmar(f, dy, gy);
+            gz = fmar(f, dz, gz);
         }
     }
 };
 
-template <typename Real>
+template <typename Real, int CAP>
 __global__ void __launch_bounds__(64) k_gradp(Grid g, const int* __restrict__ cs, StateView<Real> a,
                                               StateView<Real> b, Thresh<Real> th, StepConst kc,
                                               DevScalars* __restrict__ sc)
 {
-    __shared__ Cand<Real> buf[Cap<Real>::v];
+    __shared__ Cand<Real> buf[CAP];
     const int cell = xcd_cell(g.ncells);
     if (cell >= g.ncells) return;
     const int s0 = cs[cell], s1 = cs[cell + 1];
@@ -322,13 +348,13 @@ __global__ void __launch_bounds__(64) k_gradp(Grid g, const int* __restrict__ cs
         Real lo[3], hi[3];
         wave_box(op.active, op.xi, op.yi, op.zi, lo, hi);
         if (ballot(op.active)) {
-            StageResult r = stage(g, cs, cell, lo, hi, R2, buf, op);
+            StageResult r = stage<CAP>(g, cs, cell, lo, hi, R2, buf, op);
             op.pass = 1;
             if (!r.overflowed) {
                 op.process(buf, r.count);
             } else {
                 __syncthreads();
-                stage(g, cs, cell, lo, hi, R2, buf, op);
+                stage<CAP>(g, cs, cell, lo, hi, R2, buf, op);
             }
         }
         if (in) {
@@ -351,31 +377,43 @@ __global__ void __launch_bounds__(64) k_gradp(Grid g, const int* __restrict__ cs
 }
 
 
+// k.wave_cap selects the LDS batch (256 or 512 float candidates): the smaller
+// batch doubles the resident wavefronts per CU at the price of more batches.
+#define MPS_WAVE_LAUNCH(kern, ...)                                                                          \
+    do {                                                                                                   \
+        if (k.wave_cap == 512)                                                                             \
+            hipLaunchKernelGGL((kern<Real, cap_of<Real>(512)>), dim3(xcd_grid(g.ncells)), dim3(64), 0, st, \
+                               __VA_ARGS__);                                                               \
+        else                                                                                               \
+            hipLaunchKernelGGL((kern<Real, cap_of<Real>(256)>), dim3(xcd_grid(g.ncells)), dim3(64), 0, st, \
+                               __VA_ARGS__);                                                               \
+    } while (0)
+
 template <typename Real>
 void launch_pnd(const Grid& g, const int* cs, StateView<Real> s, const Thresh<Real>& th, const StepConst& k,
                 DevScalars* sc, int all_types, int with_pressure, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_pnd<Real>, dim3(xcd_grid(g.ncells)), dim3(64), 0, st, g, cs, s, th, sc,
-                       k.rho * k.cs * k.cs, all_types, with_pressure);
+    MPS_WAVE_LAUNCH(k_pnd, g, cs, s, th, sc, k.rho * k.cs * k.cs, all_types, with_pressure);
 }
 template <typename Real>
 void launch_visc(const Grid& g, const int* cs, StateView<Real> a, StateView<Real> b, const Thresh<Real>& th,
                  const StepConst& k, DevScalars* sc, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_visc<Real>, dim3(xcd_grid(g.ncells)), dim3(64), 0, st, g, cs, a, b, th, k, sc);
+    MPS_WAVE_LAUNCH(k_visc, g, cs, a, b, th, k, sc);
 }
 template <typename Real>
 void launch_collision(const Grid& g, const int* cs, StateView<Real> a, StateView<Real> b, const Thresh<Real>& th,
                       const StepConst& k, DevScalars* sc, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_collision<Real>, dim3(xcd_grid(g.ncells)), dim3(64), 0, st, g, cs, a, b, th, k, sc);
+    MPS_WAVE_LAUNCH(k_collision, g, cs, a, b, th, k, sc);
 }
 template <typename Real>
 void launch_gradp(const Grid& g, const int* cs, StateView<Real> a, StateView<Real> b, const Thresh<Real>& th,
                   const StepConst& k, DevScalars* sc, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_gradp<Real>, dim3(xcd_grid(g.ncells)), dim3(64), 0, st, g, cs, a, b, th, k, sc);
+    MPS_WAVE_LAUNCH(k_gradp, g, cs, a, b, th, k, sc);
 }
+#undef MPS_WAVE_LAUNCH
 
 #define MPS_INST_W(R)                                                                                          \
     template void launch_pnd<R>(const Grid&, const int*, StateView<R>, const Thresh<R>&, const StepConst&,   \
