@@ -484,12 +484,15 @@ extern "C" int mps_create(const mps_params* p, int np, int nb2, int nb1, const d
     k.dim = p->dim;
     k.gradp_skip_dummy = p->gradp_skip_dummy;
     {
-        // sweep kernels: MPS_SWEEP=tile selects the eight-particle-tile variant
+        // sweep kernels: one wavefront per cell by default; MPS_SWEEP=tile selects
+        // the eight-lanes-per-particle variant
         const char* v = getenv("MPS_SWEEP");
         k.variant = (v && !strcmp(v, "tile")) ? 1 : 0;
-        // MPS_WAVE_CAP=512 selects the larger LDS batch of the one-wavefront sweeps
+        // LDS batch of the one-wavefront sweeps (float candidates): 128 by
+        // default (4 KiB: the most resident wavefronts per CU); MPS_WAVE_CAP=256|512
         const char* wc = getenv("MPS_WAVE_CAP");
-        k.wave_cap = (wc && atoi(wc) == 512) ? 512 : 256;
+        const int cap = wc ? atoi(wc) : 128;
+        k.wave_cap = (cap == 256 || cap == 512) ? cap : 128;
     }
     double fl2 = 0.0, st2 = 0.0;
     for (int i = 0; i < np; i++) {

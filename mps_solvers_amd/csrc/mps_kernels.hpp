@@ -37,7 +37,7 @@ struct StepConst {
     int dim;
     int gradp_skip_dummy;
     int variant;   // neighbour-sweep kernels: 0 one wavefront per cell, 1 eight-particle tiles
-    int wave_cap;  // one-wavefront sweeps: LDS batch in float candidates (256 or 512)
+    int wave_cap;  // one-wavefront sweeps: LDS batch in float candidates (128, 256 or 512)
 };
 
 namespace wv {

@@ -525,7 +525,7 @@ template <typename Real>
 void launch_pnd(const Grid& g, const int* cs, StateView<Real> s, const Thresh<Real>& th,
                 const StepConst& k, DevScalars* sc, int all_types, int with_pressure, hipStream_t st)
 {
-    if (k.variant == 0) { wv::launch_pnd<Real>(g, cs, s, th, k, sc, all_types, with_pressure, st); return; }
+    if (k.variant != 1) { wv::launch_pnd<Real>(g, cs, s, th, k, sc, all_types, with_pressure, st); return; }
     hipLaunchKernelGGL(k_pnd<Real>, dim3(xcd_grid(g.ncells)), dim3(TB), 0, st, g, cs, s, th, sc,
                        k.rho * k.cs * k.cs, all_types, with_pressure);
 }
@@ -541,7 +541,7 @@ template <typename Real>
 void launch_visc(const Grid& g, const int* cs, StateView<Real> a, StateView<Real> b,
                  const Thresh<Real>& th, const StepConst& k, DevScalars* sc, hipStream_t st)
 {
-    if (k.variant == 0) { wv::launch_visc<Real>(g, cs, a, b, th, k, sc, st); return; }
+    if (k.variant != 1) { wv::launch_visc<Real>(g, cs, a, b, th, k, sc, st); return; }
     hipLaunchKernelGGL(k_visc<Real>, dim3(xcd_grid(g.ncells)), dim3(TB), 0, st, g, cs, a, b, th, k, sc);
 }
 
@@ -549,7 +549,7 @@ template <typename Real>
 void launch_collision(const Grid& g, const int* cs, StateView<Real> a, StateView<Real> b,
                       const Thresh<Real>& th, const StepConst& k, DevScalars* sc, hipStream_t st)
 {
-    if (k.variant == 0) { wv::launch_collision<Real>(g, cs, a, b, th, k, sc, st); return; }
+    if (k.variant != 1) { wv::launch_collision<Real>(g, cs, a, b, th, k, sc, st); return; }
     hipLaunchKernelGGL(k_collision<Real>, dim3(xcd_grid(g.ncells)), dim3(TB), 0, st, g, cs, a, b, th, k, sc);
 }
 
@@ -557,7 +557,7 @@ template <typename Real>
 void launch_gradp(const Grid& g, const int* cs, StateView<Real> a, StateView<Real> b,
                   const Thresh<Real>& th, const StepConst& k, DevScalars* sc, hipStream_t st)
 {
-    if (k.variant == 0) { wv::launch_gradp<Real>(g, cs, a, b, th, k, sc, st); return; }
+    if (k.variant != 1) { wv::launch_gradp<Real>(g, cs, a, b, th, k, sc, st); return; }
     hipLaunchKernelGGL(k_gradp<Real>, dim3(xcd_grid(g.ncells)), dim3(TB), 0, st, g, cs, a, b, th, k, sc);
 }
 

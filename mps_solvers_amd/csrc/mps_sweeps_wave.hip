@@ -1,9 +1,9 @@
-// mps_sweeps_wave.hip -- the "wave" variant of the neighbour sweeps: one
+// mps_sweeps_wave.hip -- the one-wavefront-per-cell neighbour sweeps: one
 // 64-lane wavefront per cell (chunks of 64 particles, lane = particle), the
 // 27-cell candidate set filtered against the wave's bounding box and staged
-// through LDS, each staged candidate broadcast to all lanes.  Same operator
-// semantics as mps_sweeps.hip (the "tile" variant); which one runs is chosen
-// per context (StepConst::variant).
+// through LDS in batches, each staged candidate broadcast to all lanes
+// (StepConst::variant 0, the default).  Same operator semantics as
+// mps_sweeps.hip (variant 1, eight lanes per particle).
 #include <float.h>
 
 #include "mps_device.hpp"
@@ -18,7 +18,7 @@ namespace wv {
 // candidates to a per-lane list (slot-major in LDS, so that every lane's k-th
 // entry sits in its own bank), then the FP64 weight runs over the dense
 // lists only.  Each lane still adds its neighbours in ascending staged order.
-constexpr int PND_LMAX = 48;   // list slots per lane (drained before any lane can overflow)
+constexpr int PND_LMAX = 32;   // list slots per lane (drained before any lane can overflow)
 
 template <typename Real>
 struct OpPnd {
@@ -59,17 +59,19 @@ struct OpPnd {
             }
         }
     }
+    __device__ __forceinline__ void test(const CandP<Real> c, int k, int& m)
+    {
+        const Real rr = rr3(c.x - xi, c.y - yi, c.z - zi);
+        if (active && rr < tn && c.j != i) {
+            L[m * WAVE + threadIdx.x] = (uint16_t)k;
+            m++;
+        }
+    }
     __device__ void process(const CandP<Real>* buf, int cnt)
     {
-        const int lane = threadIdx.x;
         int m = 0;
         for (int k = 0; k < cnt; k++) {
-            const CandP<Real> c = buf[k];
-            const Real rr = rr3(c.x - xi, c.y - yi, c.z - zi);
-            if (active && rr < tn && c.j != i) {
-                L[m * WAVE + lane] = (uint16_t)k;
-                m++;
-            }
+            test(buf[k], k, m);
             // at most 16 appends between checks: no lane passes PND_LMAX
             if ((k & 15) == 15 && ballot(m > PND_LMAX - 16)) {
                 drain(buf, m);
@@ -130,18 +132,19 @@ struct OpVisc {
         c.x = x[j]; c.y = y[j]; c.z = z[j]; c.j = j;
         c.a = u[j]; c.b = v[j]; c.c = w[j];
     }
+    __device__ __forceinline__ void visit(const Cand<Real> c)
+    {
+        const Real dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
+        const Real rr = rr3(dx, dy, dz);
+        const bool inside = (rr < tl) & (c.j != i);
+        const Real wt = inside ? fmar(re, rsq(rr), (Real)-1) : (Real)0;
+        lx = fmar(wt, c.a - ui, lx);
+        ly = fmar(wt, c.b - vi, ly);
+        lz = fmar(wt, c.c - wi, lz);
+    }
     __device__ void process(const Cand<Real>* buf, int cnt)
     {
-        for (int k = 0; k < cnt; k++) {
-            const Cand<Real> c = buf[k];
-            const Real dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
-            const Real rr = rr3(dx, dy, dz);
-            const bool inside = rr < tl && c.j != i;
-            const Real wt = inside ? fmar(re, rsq(rr), (Real)-1) : (Real)0;
-            lx = fmar(wt, c.a - ui, lx);
-            ly = fmar(wt, c.b - vi, ly);
-            lz = fmar(wt, c.c - wi, lz);
-        }
+        for (int k = 0; k < cnt; k++) visit(buf[k]);
     }
 };
 
@@ -172,7 +175,9 @@ __global__ void __launch_bounds__(64) k_visc(Grid g, const int* __restrict__ cs,
         }
         Real lo[3], hi[3];
         wave_box(op.active, op.xi, op.yi, op.zi, lo, hi);
-        if (ballot(op.active)) stage<CAP>(g, cs, cell, lo, hi, R2, buf, op);
+        if (ballot(op.active)) {
+            stage<CAP>(g, cs, cell, lo, hi, R2, buf, op);
+        }
         if (in) {
             Real xo = op.xi, yo = op.yi, zo = op.zi, uo = op.ui, vo = op.vi, wo = op.wi;
             if (op.active) {
@@ -203,29 +208,29 @@ struct OpCol {
         c.x = x[j]; c.y = y[j]; c.z = z[j]; c.j = j;
         c.a = u[j]; c.b = v[j]; c.c = w[j];
     }
-    __device__ void process(const Cand<Real>* buf, int cnt)
+    __device__ __forceinline__ void visit(const Cand<Real> c)
     {
-        for (int k = 0; k < cnt; k++) {
-            const Cand<Real> c = buf[k];
-            const Real dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
-            const Real rr = rr3(dx, dy, dz);
-            if (rr < tc && c.j != i) {
-                // v = 0.5 (u_j - u_i) . r / |r|; the sign decides, formed as the reference forms it
-                const Real hx = (Real)0.5 * (c.a - ui), hy = (Real)0.5 * (c.b - vi),
-                           hz = (Real)0.5 * (c.c - wi);
-                Real dot = hx * dx;
-                dot = dot + hy * dy;
-                dot = dot + hz * dz;
-                if (dot <= (Real)0) {
-                    const Real ods = rsq(rr);
-                    const Real f = rate * (dot * ods) * ods;
-                    cx = fmar(f, dx, cx);
-                    cy = fmar(f, dy, cy);
-                    cz = fmar(f, dz, cz);
-                    hits++;
-                }
+        const Real dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
+        const Real rr = rr3(dx, dy, dz);
+        if (rr < tc && c.j != i) {
+            // v = 0.5 (u_j - u_i) . r / |r|; the sign decides, formed as the reference forms it
+            const Real hx = (Real)0.5 * (c.a - ui), hy = (Real)0.5 * (c.b - vi), hz = (Real)0.5 * (c.c - wi);
+            Real dot = hx * dx;
+            dot = dot + hy * dy;
+            dot = dot + hz * dz;
+            if (dot <= (Real)0) {
+                const Real ods = rsq(rr);
+                const Real f = rate * (dot * ods) * ods;
+                cx = fmar(f, dx, cx);
+                cy = fmar(f, dy, cy);
+                cz = fmar(f, dz, cz);
+                hits++;
             }
         }
+    }
+    __device__ void process(const Cand<Real>* buf, int cnt)
+    {
+        for (int k = 0; k < cnt; k++) visit(buf[k]);
     }
 };
 
@@ -258,7 +263,9 @@ __global__ void __launch_bounds__(64) k_collision(Grid g, const int* __restrict_
         }
         Real lo[3], hi[3];
         wave_box(op.active, op.xi, op.yi, op.zi, lo, hi);
-        if (ballot(op.active)) stage<CAP>(g, cs, cell, lo, hi, R2, buf, op);
+        if (ballot(op.active)) {
+            stage<CAP>(g, cs, cell, lo, hi, R2, buf, op);
+        }
         if (in) {
             Real cx = op.active ? op.cx : (Real)0, cy = op.active ? op.cy : (Real)0,
                  cz = op.active ? op.cz : (Real)0;
@@ -289,29 +296,30 @@ struct OpGrad {
         c.x = x[j]; c.y = y[j]; c.z = z[j]; c.j = j;
         c.t = type[j]; c.a = p[j];
     }
+    // p_hat = min(p_i, p_j : type_j != 2, |r|^2 <= R0^2)
+    __device__ __forceinline__ void visit_min(const Cand<Real> c)
+    {
+        const Real rr = rr3(c.x - xi, c.y - yi, c.z - zi);
+        const bool near = (c.t != 2) & (rr <= re2) & (c.a < ph);
+        ph = near ? c.a : ph;
+    }
+    __device__ __forceinline__ void visit_grad(const Cand<Real> c)
+    {
+        const Real dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
+        const Real rr = rr3(dx, dy, dz);
+        const bool inside = (rr < tn) & (c.j != i) & !(skip_dummy & (c.t == 2));
+        const Real ri = rsq(rr);
+        const Real f = inside ? (c.a - ph) * (ri * ri) * fmar(re, ri, (Real)-1) : (Real)0;
+        gx = fmar(f, dx, gx);
+        gy = fmar(f, dy, gy);
+        gz = fmar(f, dz, gz);
+    }
     __device__ void process(const Cand<Real>* buf, int cnt)
     {
         if (pass == 0) {
-            // p_hat = min(p_i, p_j : type_j != 2, |r|^2 <= R0^2)
-            for (int k = 0; k < cnt; k++) {
-                const Real cxk = buf[k].x, cyk = buf[k].y, czk = buf[k].z, pk = buf[k].a;
-                const int tk = buf[k].t;
-                const Real rr = rr3(cxk - xi, cyk - yi, czk - zi);
-                const bool near = tk != 2 && rr <= re2 && pk < ph;
-                ph = near ? pk : ph;
-            }
-            return;
-        }
-        for (int k = 0; k < cnt; k++) {
-            const Cand<Real> c = buf[k];
-            const Real dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
-            const Real rr = rr3(dx, dy, dz);
-            const bool inside = rr < tn && c.j != i && !(skip_dummy && c.t == 2);
-            const Real ri = rsq(rr);
-            const Real f = inside ? (c.a - ph) * (ri * ri) * fmar(re, ri, (Real)-1) : (Real)0;
-            gx = fmar(f, dx, gx);
-            gy = fmar(f, dy, gy);
-            gz = fmar(f, dz, gz);
+            for (int k = 0; k < cnt; k++) visit_min(buf[k]);
+        } else {
+            for (int k = 0; k < cnt; k++) visit_grad(buf[k]);
         }
     }
 };
@@ -351,6 +359,7 @@ __global__ void __launch_bounds__(64) k_gradp(Grid g, const int* __restrict__ cs
             StageResult r = stage<CAP>(g, cs, cell, lo, hi, R2, buf, op);
             op.pass = 1;
             if (!r.overflowed) {
+                __syncthreads();
                 op.process(buf, r.count);
             } else {
                 __syncthreads();
@@ -377,15 +386,18 @@ __global__ void __launch_bounds__(64) k_gradp(Grid g, const int* __restrict__ cs
 }
 
 
-// k.wave_cap selects the LDS batch (256 or 512 float candidates): the smaller
+// k.wave_cap selects the LDS batch (128, 256 or 512 float candidates): the smaller
 // batch doubles the resident wavefronts per CU at the price of more batches.
 #define MPS_WAVE_LAUNCH(kern, ...)                                                                          \
     do {                                                                                                   \
         if (k.wave_cap == 512)                                                                             \
             hipLaunchKernelGGL((kern<Real, cap_of<Real>(512)>), dim3(xcd_grid(g.ncells)), dim3(64), 0, st, \
                                __VA_ARGS__);                                                               \
-        else                                                                                               \
+        else if (k.wave_cap == 256)                                                                        \
             hipLaunchKernelGGL((kern<Real, cap_of<Real>(256)>), dim3(xcd_grid(g.ncells)), dim3(64), 0, st, \
+                               __VA_ARGS__);                                                               \
+        else                                                                                               \
+            hipLaunchKernelGGL((kern<Real, cap_of<Real>(128)>), dim3(xcd_grid(g.ncells)), dim3(64), 0, st, \
                                __VA_ARGS__);                                                               \
     } while (0)
 
