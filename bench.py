@@ -76,9 +76,11 @@ def cpu_baseline(w, nb2, nb1, pos, budget_s=20.0):
 
 # Unperturbed lattices for the semi-implicit runs: n0 is the largest initial
 # number density (as the reference's calOnce takes it), so on a perturbed
-# lattice nearly every particle starts below n0, the textbook source term is
+# lattice nearly every particle starts below n0, the source term is
 # negative almost everywhere and p = max(x, 0) stays zero.
 PPE_JITTER = 0.0
+# compressibility of the Poisson matrix: alpha = 1 / (rho c^2) (calOnce), rho 1000, c 20
+PPE_ALPHA = 1.0 / (1000.0 * 20.0 ** 2)
 PPE_SCENES = {
     # config 2: same scene family at ~0.5M fluid (64 x 64 x 128), re_lap = 3.1 l0
     "eval": dict(fluid=(64, 64, 128), inner=(96, 64, 134), dp=0.0025, seed=2),
@@ -105,9 +107,9 @@ def ppe_bench(mps, spin_steps=20, train_iters=2000, log=print):
         nb2, nb1, pos = mps.dambreak(sc["fluid"], sc["inner"], 2, 2, sc["dp"], PPE_JITTER, sc["seed"])
         s = mps.Solver(ppe_params(mps, sc["dp"]), nb2, nb1, pos)
         s.mlp_init(0)
-        s.step_semi(10, alpha=0.0, x0_kind=1, tol=1e-8)
+        s.step_semi(10, alpha=PPE_ALPHA, x0_kind=1, tol=1e-8)
         s.mlp_set_record(True)
-        s.step_semi(spin_steps, alpha=0.0, x0_kind=1, tol=1e-8)
+        s.step_semi(spin_steps, alpha=PPE_ALPHA, x0_kind=1, tol=1e-8)
         X, T = s.mlp_dataset()
         Xs.append(X)
         Ts.append(T)
@@ -122,14 +124,14 @@ def ppe_bench(mps, spin_steps=20, train_iters=2000, log=print):
     l0, l1 = s.mlp_train(train_iters, batch=4096, lr=1e-3, seed=0)
     train_s = time.perf_counter() - t1
     log(f"[bench] trained {train_iters} Adam steps in {train_s:.1f} s: loss {l0:.3e} -> {l1:.3e}")
-    s.step_semi(10 + spin_steps, alpha=0.0, x0_kind=1, tol=1e-8)
+    s.step_semi(10 + spin_steps, alpha=PPE_ALPHA, x0_kind=1, tol=1e-8)
     # the system of the next step, built as mps_step_semi builds it
     s.stage_dt()
     s.stage_sort()
     s.stage_visc()
     s.stage_collision()
     s.stage_pnd()
-    nnz = s.ppe_assemble(0.0)
+    nnz = s.ppe_assemble(PPE_ALPHA)
     n = len(s.ppe_get()["b"])
     res = {}
     for name, mixed, kind in (("plain_zero", True, 0), ("prev_step", True, 1), ("learned", True, 3),
@@ -145,8 +147,9 @@ def ppe_bench(mps, spin_steps=20, train_iters=2000, log=print):
                 best = {"ms": round(ms, 3), "iters": it, "relres": rel}
         res[name] = best
     s.close()
-    return {"workload": "dam break 0.52M fluid (64x64x128, l0=2.5mm), re_lap=3.1 l0, textbook PPE "
-                        f"(alpha=0), after {10 + spin_steps} semi-implicit steps of dt=2.5e-4 s",
+    return {"workload": "dam break 0.52M fluid (64x64x128, l0=2.5mm), re_lap=3.1 l0, PPE "
+                        f"(alpha = 1/(rho c^2) as MPS_CPU_EXPL), after {10 + spin_steps} semi-implicit "
+                        "steps of dt=2.5e-4 s",
             "rows": n, "nnz": int(nnz), "solves": res,
             "learned_over_plain_time": res["learned"]["ms"] / res["plain_zero"]["ms"],
             "learned_over_prev_time": res["learned"]["ms"] / res["prev_step"]["ms"],
@@ -175,6 +178,23 @@ def traffic_from_profile(dom, slug):
         return None
     for line in open(tf):
         if line.startswith("bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+# VALU issue peak: 157.3 TFLOP/s FP32 vector / (64 lanes x 2 flop per FMA)
+VALU_PEAK_INSTR_S = 157.3e12 / 128
+
+
+def valu_from_profile(dom, slug):
+    """VALU wave64 instructions per launch of kernel `dom` (SQ_INSTS_VALU) from
+    the committed counter summary, or None."""
+    tf = os.path.join(HERE, "profiles", f"valu_step_{slug}.txt")
+    if not os.path.exists(tf):
+        return None
+    key = f"valu_instr_per_launch_{dom}:"
+    for line in open(tf):
+        if line.startswith(key):
             return float(line.split(":", 1)[1])
     return None
 
@@ -225,6 +245,12 @@ def main():
     dom = max(("visc", "collision", "pnd_pres", "gradp", "sort"), key=lambda k: per[k])
     ach = ALG_BYTES[dom] * N / (per[dom] * 1e-3) / 1e9
     traffic = traffic_from_profile(dom, w["slug"])
+    vi = valu_from_profile(dom, w["slug"])
+    valu = None
+    if vi is not None:
+        va = vi / (per[dom] * 1e-3)
+        valu = {"instr_per_launch": vi, "achieved": va, "peak": VALU_PEAK_INSTR_S, "unit": "wave64 instr/s",
+                "frac": va / VALU_PEAK_INSTR_S}
     sc = s.scalars()
     out = {
         "metric": "particle-updates/s at 1M particles; PPE ms/solve to 1e-8 residual, learned vs plain",
@@ -245,7 +271,10 @@ def main():
                    "step_roofline_frac": STEP_BYTES * N / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS,
                    "stage_ms": {k: round(v, 4) for k, v in per.items()}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic},
+                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                     # the sweeps issue far more VALU work than their bytes need: the VALU
+                     # issue rate is what bounds them (SQ_INSTS_VALU from the profile)
+                     "valu": valu},
     }
     s.close()
     if not a.no_ppe:

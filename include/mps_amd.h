@@ -73,6 +73,15 @@ int mps_geo_write(const char* path, int np, int nb2, int nb1, const double* pos,
 int mps_scene_dambreak(int dim, int fx, int fy, int fz, int nx, int ny, int nz, int layers_wall,
                        int layers_dummy, double dp, double jitter, unsigned long long seed,
                        int* np, int* nb2, int* nb1, double* pos);
+/* Seeded 3-D wave tank (no reference generator; the reference's MOTION
+ * class, MPS_CAL_/MOTION.cpp:11-16, is an empty stub): water nx*ny*depth in an
+ * open-top tank of nx*ny*nz inner sites whose x = 0 wall is a piston that may
+ * travel +-travel sites; floor and side walls reach travel sites behind it.
+ * piston (np entries, may be NULL) flags the piston's wall and dummy layers.
+ * Same layering, jitter and block order as mps_scene_dambreak. */
+int mps_scene_wavetank(int nx, int ny, int nz, int depth, int travel, int layers_wall, int layers_dummy,
+                       double dp, double jitter, unsigned long long seed, int* np, int* nb2, int* nb1,
+                       double* pos, unsigned char* piston);
 
 /* ------------------------------------------------------------------ context */
 
@@ -107,6 +116,19 @@ int mps_get_scalars(mps_ctx* c, double* out10);
 int mps_get_grid(mps_ctx* c, double* geom4, int* dims4);
 /* MPS_GPU::writeCase (MPS_CAL_expl_v1.0/MPS_GPU.cpp:138-152). */
 int mps_write_out(mps_ctx* c, const char* path);
+/* The legacy-VTK particle file of MPS_POST (MPS_POST/MPS_POST.cpp:93-172:
+ * WriteHead, WritePoint, WriteCell1, WriteAttribute) for the current state,
+ * particles in id order. */
+int mps_write_vtk(mps_ctx* c, const char* path);
+
+/* Prescribed motion (the role of MOTION::doMotion, an empty stub in
+ * MPS_CAL_/MOTION.cpp:11-16): the particles flagged in moving (np entries,
+ * input order = particle id) follow x_axis(t) = x_axis(now) + A (sin(w t) -
+ * sin(w now)), w = 2 pi freq, with velocity A w cos(w t) along axis and zero
+ * across it.  Applied at the start of every step, right after dt, for t =
+ * the step's end time, before any sweep.  amplitude 0 or moving == NULL
+ * switches it off. */
+int mps_set_motion(mps_ctx* c, const unsigned char* moving, int axis, double amplitude, double freq);
 
 /* ---------------------------------------------- per-stage entry points (parity) */
 

@@ -75,6 +75,10 @@ def lib():
         L.mps_get_scalars.argtypes = [vp, D]
         L.mps_get_grid.argtypes = [vp, D, I]
         L.mps_write_out.argtypes = [vp, C.c_char_p]
+        L.mps_write_vtk.argtypes = [vp, C.c_char_p]
+        L.mps_set_motion.argtypes = [vp, C.c_void_p, C.c_int, C.c_double, C.c_double]
+        L.mps_scene_wavetank.argtypes = [C.c_int] * 7 + [C.c_double, C.c_double, C.c_ulonglong, I, I, I, D,
+                                                         C.c_void_p]
         L.mps_stage_sort.argtypes = [vp, I, I]
         L.mps_set_dt.argtypes = [vp, C.c_double]
         L.mps_neighbor_list.argtypes = [vp, C.c_int, I, I, C.c_long, C.POINTER(C.c_long)]
@@ -188,6 +192,20 @@ def dambreak(fluid, inner, layers_wall=2, layers_dummy=2, dp=0.01, jitter=0.01, 
     return a.value, b.value, pos
 
 
+def wavetank(nx, ny, nz, depth, travel, layers_wall=2, layers_dummy=2, dp=0.004, jitter=0.01, seed=0):
+    """Seeded 3-D wave tank with a piston wall (mps_scene_wavetank): returns
+    (nb2, nb1, pos[np, 3], piston[np] as uint8 flags)."""
+    L = lib()
+    n, a, b = C.c_int(), C.c_int(), C.c_int()
+    args = [nx, ny, nz, depth, travel, layers_wall, layers_dummy, dp, jitter, seed]
+    _check(L.mps_scene_wavetank(*args, C.byref(n), C.byref(a), C.byref(b), None, None), "mps_scene_wavetank")
+    pos = np.zeros((n.value, 3))
+    piston = np.zeros(n.value, np.uint8)
+    _check(L.mps_scene_wavetank(*args, C.byref(n), C.byref(a), C.byref(b), _dp(pos),
+                                piston.ctypes.data_as(C.c_void_p)), "mps_scene_wavetank")
+    return a.value, b.value, pos, piston
+
+
 class Solver:
     """One device context (mps_create ... mps_destroy)."""
 
@@ -259,6 +277,21 @@ class Solver:
 
     def write_out(self, path):
         _check(lib().mps_write_out(self.h, os.fsencode(path)), "mps_write_out")
+
+    def write_vtk(self, path):
+        _check(lib().mps_write_vtk(self.h, os.fsencode(path)), "mps_write_vtk")
+
+    def set_motion(self, moving, axis=0, amplitude=0.05, freq=0.8):
+        """Prescribed sinusoidal motion of the flagged particles (flags by
+        particle id, e.g. the piston of wavetank()); moving=None switches it off."""
+        if moving is None:
+            _check(lib().mps_set_motion(self.h, None, int(axis), 0.0, 0.0), "mps_set_motion")
+            return
+        m = np.ascontiguousarray(moving, np.uint8)
+        if m.shape != (self.nt,):
+            raise ValueError("moving needs one flag per particle")
+        _check(lib().mps_set_motion(self.h, m.ctypes.data_as(C.c_void_p), int(axis), float(amplitude),
+                                    float(freq)), "mps_set_motion")
 
     # -------------------------------------------------------------- stages
     def stage_sort(self):

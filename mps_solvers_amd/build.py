@@ -19,8 +19,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libmps_amd.so")
-SOURCES = ["mps_sort.hip", "mps_sweeps.hip", "mps_sweeps_wave.hip", "mps_ppe.hip", "mps_mlp.hip", "mps_capi.hip",
-           "mps_ppe_capi.hip", "mps_host.cpp"]
+SOURCES = ["mps_sort.hip", "mps_sweeps.hip", "mps_sweeps_wave.hip", "mps_ppe.hip", "mps_mlp.hip", "mps_motion.hip",
+           "mps_capi.hip", "mps_ppe_capi.hip", "mps_host.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",

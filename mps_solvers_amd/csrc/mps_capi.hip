@@ -207,6 +207,61 @@ extern "C" int mps_scene_dambreak(int dim, int fx, int fy, int fz, int nx, int n
     return MPS_OK;
 }
 
+// Wave tank (3-D, lattice units of dp): water x in [0, nx), y in [0, ny),
+// z in [0, depth) in an open-top tank of inner nx x ny x nz.  The left wall is
+// a piston: its layers (x < 0 over the inner y, z range) are flagged in
+// `piston` and may travel `travel` lattice steps either way, so the floor and
+// the side walls extend `travel` steps behind it.  Layers as in
+// mps_scene_dambreak (wall nearest the water, dummies outside); particles come
+// out as dummy, wall, fluid blocks.
+extern "C" int mps_scene_wavetank(int nx, int ny, int nz, int depth, int travel, int layers_wall,
+                                  int layers_dummy, double dp, double jitter, unsigned long long seed,
+                                  int* np, int* nb2, int* nb1, double* pos, unsigned char* piston)
+{
+    if (nx <= 0 || ny <= 0 || nz <= 0 || depth < 0 || depth > nz || travel < 0 || layers_wall < 0 ||
+        layers_dummy < 0 || !np || !nb2 || !nb1)
+        return MPS_ERR_ARG;
+    const int L = layers_wall + layers_dummy;
+    long cnt[3] = {0, 0, 0};
+    for (int pass = 0; pass < 2; pass++) {
+        long off[3] = {cnt[2] + cnt[1], cnt[2], 0};   // fluid, wall, dummy start
+        long k_out[3] = {0, 0, 0};
+        if (pass == 1 && !pos) break;
+        for (int k = -L; k < nz; k++)
+            for (int j = -L; j < ny + L; j++)
+                for (int i = -L - travel; i < nx + L; i++) {
+                    const bool in_y = j >= 0 && j < ny, in_z = k >= 0;
+                    const int dj = j < 0 ? -j : (j >= ny ? j - ny + 1 : 0);
+                    const int dk = k < 0 ? -k : 0;
+                    int d, t;
+                    bool pist = false;
+                    if (i >= 0 && i < nx && in_y && in_z) {          // inside the tank
+                        if (k >= depth) continue;
+                        d = 0;
+                    } else if (i < 0 && in_y && in_z) {              // piston or the space behind it
+                        if (i < -L) continue;
+                        d = -i;
+                        pist = true;
+                    } else {                                         // floor, side walls, far wall
+                        const int di = i >= nx ? i - nx + 1 : 0;
+                        d = std::max(di, std::max(dj, dk));
+                    }
+                    t = d == 0 ? 0 : (d <= layers_wall ? 1 : 2);
+                    if (pass == 0) { cnt[t]++; continue; }
+                    const long idx = off[t] + k_out[t]++;
+                    double q[3] = {i * dp, j * dp, k * dp};
+                    for (int a = 0; a < 3; a++)
+                        q[a] += jitter * dp * jitter_unit(seed, 3ull * (unsigned long long)idx + a);
+                    pos[3 * idx + 0] = q[0]; pos[3 * idx + 1] = q[1]; pos[3 * idx + 2] = q[2];
+                    if (piston) piston[idx] = pist ? 1 : 0;
+                }
+    }
+    *np = (int)(cnt[0] + cnt[1] + cnt[2]);
+    *nb2 = (int)cnt[2];
+    *nb1 = (int)cnt[1];
+    return MPS_OK;
+}
+
 // ============================================================== context ===
 
 namespace {
@@ -397,6 +452,7 @@ static int step_once(mps_ctx* c)
     tic(c, T_DT);
     launch_dt(c->kc, c->sc, c->st);
     toc(c, T_DT);
+    if ((rc = mps_ctx_motion(c))) return rc;
     const bool every = c->P.rebin_every_move != 0;
     if (!every && (rc = do_sort(c))) return rc;
     if ((rc = run_stage(c, S_VISC))) return rc;
@@ -703,6 +759,78 @@ extern "C" int mps_write_out(mps_ctx* c, const char* path)
         fprintf(f, "%d %d %lf %lf %lf %lf %lf %lf %lf ", id[i], ty[i], pos[3 * i], pos[3 * i + 1],
                 pos[3 * i + 2], vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], pr[i]);
     fclose(f);
+    return MPS_OK;
+}
+
+extern "C" int mps_write_vtk(mps_ctx* c, const char* path)
+{
+    if (!c || !path) return MPS_ERR_ARG;
+    // every particle of GEO.dat, in id order (excluded ones at their last position)
+    const int na = c->nt;
+    const size_t N = (size_t)c->nt;
+    std::vector<double> pos(3 * N), vel(3 * N), pr(N);
+    std::vector<int> id(N);
+    int rc;
+    if ((rc = mps_get_state(c, 1, pos.data(), vel.data(), pr.data(), nullptr, id.data(), nullptr))) return rc;
+    FILE* f = fopen(path, "wt");
+    if (!f) return MPS_ERR_IO;
+    fprintf(f, "# vtk DataFile Version 3.0\nunstructured grid\nASCII\nDATASET UNSTRUCTURED_GRID\n");
+    fprintf(f, "POINTS %d float\n", na);
+    for (int i = 0; i < na; i++) fprintf(f, "%lf %lf %lf ", pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+    fprintf(f, "\nCELLS %d %d\n", na, 2 * na);
+    for (int i = 0; i < na; i++) fprintf(f, "%d %d ", 1, i);
+    fprintf(f, "\nCELL_TYPES %d\n", na);
+    for (int i = 0; i < na; i++) fprintf(f, "1 ");
+    fprintf(f, "\nPOINT_DATA %d\nSCALARS id int 1\nLOOKUP_TABLE default\n", na);
+    for (int i = 0; i < na; i++) fprintf(f, "%d ", id[i]);
+    fprintf(f, "\nSCALARS pressure float 1\nLOOKUP_TABLE default\n");
+    for (int i = 0; i < na; i++) fprintf(f, "%lf ", pr[i]);
+    fprintf(f, "\nVECTORS velocity float\n");
+    for (int i = 0; i < na; i++) fprintf(f, "%lf %lf %lf ", vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]);
+    fprintf(f, "\n");
+    const bool ok = !ferror(f);
+    fclose(f);
+    return ok ? MPS_OK : MPS_ERR_IO;
+}
+
+int mps_ctx_motion(mps_ctx* c)
+{
+    if (!c->mov || c->kc.mov_amp == 0.0) return MPS_OK;
+    if (c->prec)
+        launch_motion<double>(c->nt, c->view<double>(c->kin, c->aux), c->mov, c->xref, c->mov_axis,
+                              c->kc.mov_amp, c->kc.mov_omega, c->sc, c->st);
+    else
+        launch_motion<float>(c->nt, c->view<float>(c->kin, c->aux), c->mov, c->xref, c->mov_axis,
+                             c->kc.mov_amp, c->kc.mov_omega, c->sc, c->st);
+    HIPOK(hipGetLastError());
+    return MPS_OK;
+}
+
+extern "C" int mps_set_motion(mps_ctx* c, const unsigned char* moving, int axis, double amplitude, double freq)
+{
+    if (!c || axis < 0 || axis > 2 || !std::isfinite(amplitude) || !std::isfinite(freq)) return MPS_ERR_ARG;
+    if (!moving || amplitude == 0.0) {
+        c->kc.mov_amp = 0.0;
+        return MPS_OK;
+    }
+    const size_t N = (size_t)c->nt;
+    int rc;
+    if (!c->mov) {
+        if ((rc = dmalloc(c, (void**)&c->mov, N))) return rc;
+        if ((rc = dmalloc(c, (void**)&c->xref, 8 * N))) return rc;
+    }
+    HIPOK(hipMemcpyAsync(c->mov, moving, N, hipMemcpyHostToDevice, c->st));
+    c->mov_axis = axis;
+    c->kc.mov_amp = amplitude;
+    c->kc.mov_omega = 2.0 * M_PI * freq;
+    if (c->prec)
+        launch_motion_ref<double>(c->nt, c->view<double>(c->kin, c->aux), c->mov, c->xref, axis, amplitude,
+                                  c->kc.mov_omega, c->sc, c->st);
+    else
+        launch_motion_ref<float>(c->nt, c->view<float>(c->kin, c->aux), c->mov, c->xref, axis, amplitude,
+                                 c->kc.mov_omega, c->sc, c->st);
+    HIPOK(hipGetLastError());
+    HIPOK(hipStreamSynchronize(c->st));
     return MPS_OK;
 }
 

@@ -41,6 +41,10 @@ struct mps_ctx {
     hipEvent_t ev[T_N][2] = {};
     double ms[T_N] = {};
     std::vector<void*> allocs;
+    // prescribed motion: flags and reference coordinate by particle id
+    unsigned char* mov = nullptr;
+    double* xref = nullptr;
+    int mov_axis = 0;   // amplitude and angular frequency live in kc
     PpeBuffers* ppe = nullptr;
     MlpBuffers* mlp = nullptr;
 
@@ -58,6 +62,7 @@ struct mps_ctx {
 
 
 int mps_ctx_divide(mps_ctx* c);
+int mps_ctx_motion(mps_ctx* c);   // prescribed motion for the step about to run (no-op when off)
 int mps_ctx_stage(mps_ctx* c, int stage);   // 0 visc, 1 collision, 2 pnd+p, 3 dummy p, 4 grad p, 5 pnd only
 void mps_ppe_free(mps_ctx* c);
 void mps_mlp_free(mps_ctx* c);

@@ -34,6 +34,7 @@ struct StepConst {
     double rho, nu, cs, cfl, dp, dt_max, dt_min, col_rate;
     double g[3];
     double umax2_static;   // max |u|^2 over particles that never move
+    double mov_amp, mov_omega;   // prescribed motion (amplitude 0: none); its speed enters dt
     int dim;
     int gradp_skip_dummy;
     int variant;   // neighbour-sweep kernels: 0 one wavefront per cell, 1 eight-particle tiles
@@ -89,5 +90,13 @@ void launch_calonce(const Grid& g, const int* cs, int np, StateView<Real> s, con
 template <typename Real>
 void launch_nlist(const Grid& g, const int* cs, int np, StateView<Real> s, const Thresh<Real>& th,
                   int which, int* row_ptr, int* col, int fill, hipStream_t st);
+
+// prescribed wall motion (mps_motion.hip)
+template <typename Real>
+void launch_motion_ref(int n, StateView<Real> s, const unsigned char* mov, double* xref, int axis, double amp,
+                       double omega, const DevScalars* sc, hipStream_t st);
+template <typename Real>
+void launch_motion(int n, StateView<Real> s, const unsigned char* mov, const double* xref, int axis, double amp,
+                   double omega, const DevScalars* sc, hipStream_t st);
 
 }  // namespace mps

@@ -176,7 +176,10 @@ __global__ void __launch_bounds__(TB) k_ppe_assemble(Grid g, const int* __restri
 
 // ----------------------------------------------------------------- SpMV ---
 // CSR-vector: G lanes per row, lane-strided partial sums, then a shuffle tree
-// over the G lanes (fixed order: deterministic).
+// over the G lanes (fixed order: deterministic).  Each lane issues its column
+// and value loads four entries at a time before the gathers of p, so that
+// several loads are in flight per lane; the lane still adds its entries in
+// ascending order.
 template <int G, typename MV, typename V, typename Acc>
 __device__ __forceinline__ Acc row_dot(int row, int g, const int* __restrict__ rp,
                                        const int* __restrict__ col, const MV* __restrict__ val,
@@ -184,21 +187,72 @@ __device__ __forceinline__ Acc row_dot(int row, int g, const int* __restrict__ r
 {
     Acc a = (Acc)0;
     const int k1 = rp[row + 1];
-    for (int k = rp[row] + g; k < k1; k += G) a += (Acc)val[k] * (Acc)p[col[k]];
+    int k = rp[row] + g;
+    for (; k + 3 * G < k1; k += 4 * G) {
+        const int c0 = col[k], c1 = col[k + G], c2 = col[k + 2 * G], c3 = col[k + 3 * G];
+        const MV v0 = val[k], v1 = val[k + G], v2 = val[k + 2 * G], v3 = val[k + 3 * G];
+        const V p0 = p[c0], p1 = p[c1], p2 = p[c2], p3 = p[c3];
+        a += (Acc)v0 * (Acc)p0;
+        a += (Acc)v1 * (Acc)p1;
+        a += (Acc)v2 * (Acc)p2;
+        a += (Acc)v3 * (Acc)p3;
+    }
+    for (; k < k1; k += G) a += (Acc)val[k] * (Acc)p[col[k]];
 #pragma unroll
     for (int m = G / 2; m >= 1; m >>= 1) a += __shfl_xor(a, m);
     return a;
 }
 
-// q = A p, pq = p . q
+// The inner CG runs on a fixed grid (kCgBlocks x kCgThreads, grid-stride
+// loops).  A kernel that produces a dot product writes one partial per block;
+// every block of the next kernel sums all partials itself, in the same fixed
+// order, so each block holds the identical value without atomics or a wait on
+// other blocks.  Block 0 publishes the scalars the host reads.
+constexpr int kCgBlocks = 1024;
+constexpr int kCgThreads = 512;
+
+__device__ __forceinline__ void block_partial(double v, double* __restrict__ partial)
+{
+    __shared__ double red[kCgThreads / WAVE];
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    v = wave_sum(v);
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int k = 0; k < kCgThreads / WAVE; k++) s += red[k];
+        partial[blockIdx.x] = s;
+    }
+}
+
+__device__ __forceinline__ double block_total(const double* __restrict__ partial, int np)
+{
+    __shared__ double red[kCgThreads / WAVE];
+    __shared__ double tot;
+    const int lane = threadIdx.x & (WAVE - 1), w = threadIdx.x / WAVE;
+    double s = 0.0;
+    for (int k = threadIdx.x; k < np; k += kCgThreads) s += partial[k];
+    s = wave_sum(s);
+    if (lane == 0) red[w] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < kCgThreads / WAVE; k++) t += red[k];
+        tot = t;
+    }
+    __syncthreads();
+    return tot;
+}
+
+// q = A p; partial sums of p . q
 template <int G, typename MV, typename V>
-__global__ void __launch_bounds__(256) k_spmv_dot(int n, const int* __restrict__ rp, const int* __restrict__ col,
-                                                  const MV* __restrict__ val, const V* __restrict__ p,
-                                                  V* __restrict__ q, double* __restrict__ partial,
-                                                  PpeScalars* ps)
+__global__ void __launch_bounds__(kCgThreads) k_spmv_dot(int n, const int* __restrict__ rp,
+                                                         const int* __restrict__ col, const MV* __restrict__ val,
+                                                         const V* __restrict__ p, V* __restrict__ q,
+                                                         double* __restrict__ part_pq, const PpeScalars* ps)
 {
     if (ps->done) return;
-    const int g = threadIdx.x % G, rpb = blockDim.x / G;
+    const int g = threadIdx.x % G, rpb = kCgThreads / G;
     double d = 0.0;
     for (int row = blockIdx.x * rpb + threadIdx.x / G; row < n; row += gridDim.x * rpb) {
         const V a = row_dot<G, MV, V, V>(row, g, rp, col, val, p);
@@ -207,47 +261,60 @@ __global__ void __launch_bounds__(256) k_spmv_dot(int n, const int* __restrict__
             d += (double)p[row] * (double)a;
         }
     }
-    block_sum_last(d, partial, &ps->ticket[1], &ps->pq);
+    block_partial(d, part_pq);
 }
 
-// x += alpha p, r -= alpha q, rr_new = r . r; the last block sets beta and done
+// pq = sum of the SpMV partials; x += alpha p, r -= alpha q with alpha = rr / pq;
+// partial sums of r . r.  The MPS Poisson matrix is negative definite: p.Ap < 0
+// is normal; only a vanishing or non-finite p.Ap is a breakdown.
 template <typename V>
-__global__ void __launch_bounds__(256) k_cg_update(int n, V* __restrict__ x, V* __restrict__ r,
-                                                   const V* __restrict__ p, const V* __restrict__ q,
-                                                   double* __restrict__ partial, PpeScalars* ps)
+__global__ void __launch_bounds__(kCgThreads) k_cg_update(int n, V* __restrict__ x, V* __restrict__ r,
+                                                          const V* __restrict__ p, const V* __restrict__ q,
+                                                          const double* __restrict__ part_pq,
+                                                          double* __restrict__ part_rr, PpeScalars* ps)
 {
     if (ps->done) return;
-    const double rr = ps->rr, pq = ps->pq;
+    const double pq = block_total(part_pq, gridDim.x);
+    const double rr = ps->rr;
+    if (!(fabs(pq) > 0.0 && isfinite(pq))) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) ps->done = 2;
+        return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ps->pq = pq;
+        ps->rr_old = rr;   // k_cg_dir rewrites rr while its blocks still need this one
+    }
     const V alpha = (V)(rr / pq);
     double d = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (int i = blockIdx.x * kCgThreads + threadIdx.x; i < n; i += gridDim.x * kCgThreads) {
         x[i] += alpha * p[i];
         const V ri = r[i] - alpha * q[i];
         r[i] = ri;
         d += (double)ri * (double)ri;
     }
-    double tot;
-    if (block_sum_last(d, partial, &ps->ticket[2], &tot) && threadIdx.x == 0) {
-        ps->beta = tot / rr;
-        ps->rr = tot;
-        ps->iters += 1;
-        // the MPS Poisson matrix is negative definite: p.Ap < 0 is normal; only
-        // a vanishing or non-finite p.Ap is a breakdown
-        const bool ok = fabs(pq) > 0.0 && isfinite(pq) && isfinite(tot);
-        if (!ok) ps->done = 2;
-        else if (!(tot > ps->tol2)) ps->done = 1;
-    }
+    block_partial(d, part_rr);
 }
 
-// p = r + beta p
+// rr_new = sum of the update partials; beta = rr_new / rr; p = r + beta p.
+// Block 0 publishes rr, beta, the iteration count and the stop flag.
 template <typename V>
-__global__ void __launch_bounds__(256) k_cg_dir(int n, const V* __restrict__ r, V* __restrict__ p,
-                                                const PpeScalars* __restrict__ ps)
+__global__ void __launch_bounds__(kCgThreads) k_cg_dir(int n, const V* __restrict__ r, V* __restrict__ p,
+                                                       const double* __restrict__ part_rr, PpeScalars* ps)
 {
     if (ps->done) return;
-    const V beta = (V)ps->beta;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = r[i] + beta * p[i];
+    const double tot = block_total(part_rr, gridDim.x);
+    const double rr = ps->rr_old;
+    const double beta = tot / rr;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ps->beta = beta;
+        ps->rr = tot;
+        ps->iters += 1;
+        if (!isfinite(tot)) ps->done = 2;
+        else if (!(tot > ps->tol2)) ps->done = 1;
+    }
+    const V bv = (V)beta;
+    for (int i = blockIdx.x * kCgThreads + threadIdx.x; i < n; i += gridDim.x * kCgThreads)
+        p[i] = r[i] + bv * p[i];
 }
 
 // Outer residual in FP64: r64 = b - A64 x; rr64 = r64 . r64.  Also starts the
@@ -314,7 +381,7 @@ static inline int red_grid(long threads) { return std::min(cdiv(threads, 256), k
 
 // doubles in each partial buffer: the block partials of the reducing kernels,
 // and also the uint32 tile totals of the row-pointer scan (4096 rows a tile)
-int ppe_partial_blocks(int n) { return std::max(kRedBlocks, cdiv((long)n + 1, 2 * 4096)) + 8; }
+int ppe_partial_blocks(int n) { return std::max(std::max(kRedBlocks, kCgBlocks), cdiv((long)n + 1, 2 * 4096)) + 8; }
 
 template <typename Real>
 void launch_ppe_rows(int n, StateView<Real> s, const DevScalars* sc, double beta, double btmp_over,
@@ -336,12 +403,13 @@ void launch_ppe_assemble(const Grid& g, const int* cs, StateView<Real> s, const 
 
 template <typename MV, typename V>
 void launch_cg_iter(int n, const int* rp, const int* col, const MV* val, V* x, V* r, V* p, V* q,
-                    double* partial, PpeScalars* ps, hipStream_t st)
+                    double* partial, double* partial2, PpeScalars* ps, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_spmv_dot<kG, MV, V>), dim3(red_grid((long)n * kG)), dim3(256), 0, st, n, rp, col,
-                       val, p, q, partial, ps);
-    hipLaunchKernelGGL(k_cg_update<V>, dim3(red_grid(n)), dim3(256), 0, st, n, x, r, p, q, partial, ps);
-    hipLaunchKernelGGL(k_cg_dir<V>, dim3(cdiv(n, 256)), dim3(256), 0, st, n, r, p, ps);
+    hipLaunchKernelGGL((k_spmv_dot<kG, MV, V>), dim3(kCgBlocks), dim3(kCgThreads), 0, st, n, rp, col, val, p, q,
+                       partial, ps);
+    hipLaunchKernelGGL(k_cg_update<V>, dim3(kCgBlocks), dim3(kCgThreads), 0, st, n, x, r, p, q, partial,
+                       partial2, ps);
+    hipLaunchKernelGGL(k_cg_dir<V>, dim3(kCgBlocks), dim3(kCgThreads), 0, st, n, r, p, partial2, ps);
 }
 
 template <typename V>
@@ -379,9 +447,9 @@ template void launch_ppe_assemble<double>(const Grid&, const int*, StateView<dou
                                           const DevScalars*, double, int, int*, int*, float*, double*,
                                           hipStream_t);
 template void launch_cg_iter<float, float>(int, const int*, const int*, const float*, float*, float*,
-                                           float*, float*, double*, PpeScalars*, hipStream_t);
+                                           float*, float*, double*, double*, PpeScalars*, hipStream_t);
 template void launch_cg_iter<double, double>(int, const int*, const int*, const double*, double*,
-                                             double*, double*, double*, double*, PpeScalars*,
+                                             double*, double*, double*, double*, double*, PpeScalars*,
                                              hipStream_t);
 template void launch_residual<float>(int, const int*, const int*, const double*, const double*,
                                      const double*, float*, float*, float*, double*, double*,

@@ -11,6 +11,7 @@ namespace mps {
 
 struct PpeScalars {
     double rr, rr0, pq, beta, tol2;   // inner CG
+    double rr_old;                    // rr of the previous inner iteration
     double rr64, bb, rin;             // outer residual, |b|^2, inner start
     int done, iters;                  // done: 1 converged, 2 breakdown (p.Ap <= 0)
     unsigned int ticket[4];
@@ -78,7 +79,7 @@ void launch_ppe_assemble(const Grid& g, const int* cs, StateView<Real> s, const 
                          int* row_ptr, int* col, float* v32, double* v64, hipStream_t st);
 template <typename MV, typename V>
 void launch_cg_iter(int n, const int* rp, const int* col, const MV* val, V* x, V* r, V* p, V* q,
-                    double* partial, PpeScalars* ps, hipStream_t st);
+                    double* partial, double* partial2, PpeScalars* ps, hipStream_t st);
 template <typename V>
 void launch_residual(int n, const int* rp, const int* col, const double* val, const double* b,
                      const double* x, V* e, V* r, V* p, double* partial, double* partial2,

@@ -205,10 +205,11 @@ static int solve_t(mps_ctx* c, double tol, double tol_in, int maxit, int* iters,
             for (int k = 0; k < batch; k++) {
                 if (sizeof(V) == 4)
                     launch_cg_iter<float, float>(n, B->rp, B->col, B->v32, (float*)e, (float*)r, (float*)p,
-                                                 (float*)q, B->partial, B->ps, c->st);
+                                                 (float*)q, B->partial, B->partial2, B->ps, c->st);
                 else
                     launch_cg_iter<double, double>(n, B->rp, B->col, B->v64, (double*)e, (double*)r,
-                                                   (double*)p, (double*)q, B->partial, B->ps, c->st);
+                                                   (double*)p, (double*)q, B->partial, B->partial2, B->ps,
+                                                   c->st);
             }
             HIPOK(hipGetLastError());
             HIPOK(hipStreamSynchronize(c->st));
@@ -322,6 +323,7 @@ extern "C" int mps_step_semi(mps_ctx* c, int nsteps, double alpha, int x0_kind, 
     int rc = MPS_OK;
     for (int s = 0; s < nsteps && rc == MPS_OK; s++) {
         launch_dt(c->kc, c->sc, c->st);
+        if ((rc = mps_ctx_motion(c))) break;
         if (!every && (rc = mps_ctx_divide(c))) break;
         if ((rc = mps_ctx_stage(c, 0))) break;
         if (every && (rc = mps_ctx_divide(c))) break;

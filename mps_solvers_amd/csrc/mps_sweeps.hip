@@ -386,6 +386,11 @@ __global__ void k_dt(StepConst kc, DevScalars* sc)
     if (threadIdx.x != 0) return;
     double m2 = (double)__uint_as_float(sc->maxu2_bits);
     if (kc.umax2_static > m2) m2 = kc.umax2_static;
+    if (kc.mov_amp != 0.0) {
+        // the moving walls carry A w cos(w t) from the last motion update
+        const double va = (double)(float)(kc.mov_amp * kc.mov_omega * cos(kc.mov_omega * sc->time));
+        if (va * va > m2) m2 = va * va;
+    }
     double dt = kc.dt_max;
     if (m2 > 0.0) {
         const double t = kc.cfl * kc.dp / sqrt(m2);

@@ -65,6 +65,7 @@ def lib():
         _lib.orc_sim_get.argtypes = [C.c_void_p] + [D] * 8 + [I, I]
         _lib.orc_sim_consts.argtypes = [C.c_void_p, D]
         _lib.orc_sim_grid.argtypes = [C.c_void_p, C.POINTER(Grid)]
+        _lib.orc_sim_set_motion.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double]
     return _lib
 
 
@@ -297,6 +298,15 @@ class Sim:
         G = Grid()
         lib().orc_sim_grid(self.h, C.byref(G))
         return G
+
+    def set_motion(self, moving, axis=0, amplitude=0.05, freq=0.8):
+        """Prescribed sinusoidal motion of the particles flagged by id (None: off)."""
+        if moving is None:
+            lib().orc_sim_set_motion(self.h, None, int(axis), 0.0, 0.0)
+            return
+        self._mov = np.ascontiguousarray(moving, np.uint8)
+        lib().orc_sim_set_motion(self.h, self._mov.ctypes.data_as(C.c_void_p), int(axis), float(amplitude),
+                                 float(freq))
 
     def __del__(self):
         if getattr(self, "h", None):

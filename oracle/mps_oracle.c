@@ -688,6 +688,11 @@ struct orc_sim {
     double *x2, *y2, *z2, *u2, *v2, *w2, *tmp;
     int *type, *id, *normal, *key, *perm, *index, *cs, *itmp;
     double n0, lambda, coef, dt, time;
+    /* prescribed motion: flags and reference coordinate by particle id */
+    unsigned char* mov;
+    double* xref;
+    int mov_axis;
+    double mov_amp, mov_omega;
 };
 
 static void permute_d(int nt, const int* perm, double* a, double* tmp)
@@ -767,6 +772,8 @@ void orc_sim_destroy(orc_sim* S)
     for (int q = 0; q < 15; q++) free(dd[q]);
     int* ii[] = {S->type, S->id, S->normal, S->key, S->perm, S->index, S->itmp, S->cs};
     for (int q = 0; q < 8; q++) free(ii[q]);
+    free(S->mov);
+    free(S->xref);
     free(S);
 }
 
@@ -781,6 +788,51 @@ static void swap_state(orc_sim* S)
     t = S->w; S->w = S->w2; S->w2 = t;
 }
 
+static double st_round(const orc_sim* S, double v) { return S->P.f32 ? (double)(float)v : v; }
+
+static double* axis_of(orc_sim* S, int a, int vel)
+{
+    if (vel) return a == 0 ? S->u : (a == 1 ? S->v : S->w);
+    return a == 0 ? S->x : (a == 1 ? S->y : S->z);
+}
+
+/* Prescribed sinusoidal motion (the reference's MOTION::doMotion is an empty
+ * stub, MPS_CAL_/MOTION.cpp:11-16): flagged particles (by id) follow
+ * x_axis(t) = xref + A sin(w t), velocity A w cos(w t) along the axis, zero
+ * across it; xref is taken so that the law continues from the current state. */
+void orc_sim_set_motion(orc_sim* S, const unsigned char* moving_by_id, int axis, double amp, double freq)
+{
+    free(S->mov); free(S->xref);
+    S->mov = NULL; S->xref = NULL; S->mov_amp = 0.0;
+    if (!moving_by_id || amp == 0.0) return;
+    S->mov = (unsigned char*)malloc((size_t)S->nt);
+    S->xref = (double*)calloc((size_t)S->nt, sizeof(double));
+    memcpy(S->mov, moving_by_id, (size_t)S->nt);
+    S->mov_axis = axis; S->mov_amp = amp; S->mov_omega = 2.0 * 3.14159265358979323846 * freq;
+    double* xa = axis_of(S, axis, 0);
+    const double va = st_round(S, amp * S->mov_omega * cos(S->mov_omega * S->time));
+    for (int i = 0; i < S->nt; i++) {
+        const int o = S->id[i];
+        if (!S->mov[o]) continue;
+        S->xref[o] = xa[i] - amp * sin(S->mov_omega * S->time);
+        for (int a = 0; a < 3; a++) axis_of(S, a, 1)[i] = a == axis ? va : 0.0;
+    }
+}
+
+static void sim_motion(orc_sim* S, double t)
+{
+    if (!S->mov || S->mov_amp == 0.0) return;
+    double* xa = axis_of(S, S->mov_axis, 0);
+    const double pos_off = S->mov_amp * sin(S->mov_omega * t);
+    const double va = st_round(S, S->mov_amp * S->mov_omega * cos(S->mov_omega * t));
+    for (int i = 0; i < S->nt; i++) {
+        const int o = S->id[i];
+        if (!S->mov[o]) continue;
+        xa[i] = st_round(S, S->xref[o] + pos_off);
+        for (int a = 0; a < 3; a++) axis_of(S, a, 1)[i] = a == S->mov_axis ? va : 0.0;
+    }
+}
+
 /* One explicit step.  rebin_every_move = 0 follows MPS_CAL_/MPS_GPU.cpp:611-714
  * (one re-binning at the start of the step); 1 follows
  * MPS_CAL_expl_v1.0/MPS_GPU_EXPL.cpp:85-147 (re-binning after every position
@@ -792,6 +844,7 @@ int orc_sim_step(orc_sim* S, int nsteps, int rebin_every_move)
         int small = 0;
         S->dt = orc_update_dt(&S->P, S->np, S->u, S->v, S->w, &small);
         if (small) return 1;
+        sim_motion(S, S->time + S->dt);
         if (!rebin_every_move) sim_divide(S);
         orc_visc(&S->P, &S->G, S->np, S->x, S->y, S->z, S->u, S->v, S->w, S->type, S->cs,
                  S->coef, S->dt, S->x2, S->y2, S->z2, S->u2, S->v2, S->w2);

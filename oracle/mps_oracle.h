@@ -106,6 +106,8 @@ void orc_sim_get(const orc_sim* S, double* x, double* y, double* z, double* u, d
                  double* w, double* p, double* n, int* type, int* id);
 void orc_sim_consts(const orc_sim* S, double* out /* n0, lambda, dt, time, coef, ncells */);
 void orc_sim_grid(const orc_sim* S, orc_grid* G);
+/* Prescribed sinusoidal motion of the particles flagged by id (NULL: off). */
+void orc_sim_set_motion(orc_sim* S, const unsigned char* moving_by_id, int axis, double amp, double freq);
 
 #ifdef __cplusplus
 }
