@@ -70,6 +70,10 @@ public:
                 snprintf(name, sizeof name, "/out/%04d.out", count);
                 int rc = writeCase(dir_ + name);
                 if (rc) return rc;
+                if (vtk_) {
+                    snprintf(name, sizeof name, "/out/part%04d.vtk", count);
+                    if ((rc = writeVtk(dir_ + name))) return rc;
+                }
                 threshold += P_.tout;
                 count++;
             }
@@ -81,6 +85,16 @@ public:
 
     // MPS_GPU::writeCase (MPS_CAL_expl_v1.0/MPS_GPU.cpp:138-152)
     int writeCase(const std::string& path) { return mps_write_out(ctx_, path.c_str()); }
+
+    // MPS_POST's part%04d.vtk for the current state (MPS_POST/MPS_POST.cpp:93-172)
+    int writeVtk(const std::string& path) { return mps_write_vtk(ctx_, path.c_str()); }
+
+    // The role of MOTION::doMotion (an empty stub, MPS_CAL_/MOTION.cpp:11-16):
+    // flagged particles (by id) move as A sin(2 pi f t) along axis.
+    int setMotion(const unsigned char* moving, int axis, double amplitude, double freq)
+    {
+        return mps_set_motion(ctx_, moving, axis, amplitude, freq);
+    }
 
     // getters (MPS_CAL_/MPS_GPU.h:158-166); arrays refreshed from the device
     int getNp()
@@ -94,6 +108,9 @@ public:
     const double* getPres() { refresh(); return pr_.data(); }
     const int* getId() { refresh(); return id_.data(); }
     const int* getType() { refresh(); return type_.data(); }
+
+    // also write out/part%04d.vtk next to each .out file in mps_cal()
+    void setWriteVtk(bool on) { vtk_ = on; }
 
     double time() const { return d_time_; }
     long steps() const { return i_step_; }
@@ -125,6 +142,7 @@ protected:
     double d_time_ = 0.0;
     long i_step_ = 0;
     bool stale_ = true;
+    bool vtk_ = false;
     std::vector<double> pos_, vel_, pr_;
     std::vector<int> id_, type_;
 };

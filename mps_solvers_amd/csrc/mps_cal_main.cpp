@@ -2,7 +2,8 @@
 // (MPS_CAL_v4.0/main.cpp:14-37): read ./input.dat and ./GEO.dat, run to TT,
 // write ./out/%04d.out every TOUT.
 //
-//   mps_cal [dir] [--semi] [--fp64] [--rebin-every-move] [--steps N] [--learned]
+//   mps_cal [dir] [--semi] [--fp64] [--rebin-every-move] [--steps N] [--learned] [--vtk]
+// --vtk also writes ./out/part%04d.vtk (the files MPS_POST would make).
 #include <sys/stat.h>
 
 #include <cstdio>
@@ -15,13 +16,14 @@
 int main(int argc, char** argv)
 {
     std::string dir = ".";
-    bool semi = false, fp64 = false, every = false, learned = false;
+    bool semi = false, fp64 = false, every = false, learned = false, vtk = false;
     long steps = -1;
     for (int i = 1; i < argc; i++) {
         if (!strcmp(argv[i], "--semi")) semi = true;
         else if (!strcmp(argv[i], "--fp64")) fp64 = true;
         else if (!strcmp(argv[i], "--rebin-every-move")) every = true;
         else if (!strcmp(argv[i], "--learned")) learned = true;
+        else if (!strcmp(argv[i], "--vtk")) vtk = true;
         else if (!strcmp(argv[i], "--steps") && i + 1 < argc) steps = atol(argv[++i]);
         else dir = argv[i];
     }
@@ -29,6 +31,7 @@ int main(int argc, char** argv)
     mps_amd::MPS_GPU_EXPL expl(fp64 ? 1 : 0, every ? 1 : 0);
     mps_amd::MPS_CPU_EXPL impl(fp64 ? 1 : 0, every ? 1 : 0, 1e-8, learned ? 3 : 1);
     mps_amd::MPS_GPU_EXPL& m = semi ? (mps_amd::MPS_GPU_EXPL&)impl : expl;
+    m.setWriteVtk(vtk);
     int rc = m.Initial(dir);
     if (rc) {
         fprintf(stderr, "mps_cal: initialisation failed with status %d\n", rc);

@@ -223,6 +223,7 @@ __device__ __forceinline__ void block_partial(double v, double* __restrict__ par
         for (int k = 0; k < kCgThreads / WAVE; k++) s += red[k];
         partial[blockIdx.x] = s;
     }
+    __syncthreads();   // red[] is free again for a following call
 }
 
 __device__ __forceinline__ double block_total(const double* __restrict__ partial, int np)
@@ -321,14 +322,13 @@ __global__ void __launch_bounds__(kCgThreads) k_cg_dir(int n, const V* __restric
 // inner solve A e = r: e = 0, r_in = p = r64 (in the inner precision),
 // rr = r_in . r_in, tol2 = max(tol_in^2 rr, floor2), done = 0.
 template <int G, typename V>
-__global__ void __launch_bounds__(256) k_residual(int n, const int* __restrict__ rp, const int* __restrict__ col,
-                                                  const double* __restrict__ val, const double* __restrict__ b,
-                                                  const double* __restrict__ x, V* __restrict__ e,
-                                                  V* __restrict__ r, V* __restrict__ p,
-                                                  double* __restrict__ partial, double* __restrict__ partial2,
-                                                  PpeScalars* ps, double tol_in2, double floor2)
+__global__ void __launch_bounds__(kCgThreads) k_residual(int n, const int* __restrict__ rp, const int* __restrict__ col,
+                                                         const double* __restrict__ val, const double* __restrict__ b,
+                                                         const double* __restrict__ x, V* __restrict__ e,
+                                                         V* __restrict__ r, V* __restrict__ p,
+                                                         double* __restrict__ part64, double* __restrict__ part_in)
 {
-    const int g = threadIdx.x % G, rpb = blockDim.x / G;
+    const int g = threadIdx.x % G, rpb = kCgThreads / G;
     double d64 = 0.0, din = 0.0;
     for (int row = blockIdx.x * rpb + threadIdx.x / G; row < n; row += gridDim.x * rpb) {
         const double a = row_dot<G, double, double, double>(row, g, rp, col, val, x);
@@ -342,8 +342,18 @@ __global__ void __launch_bounds__(256) k_residual(int n, const int* __restrict__
             din += (double)rv * (double)rv;
         }
     }
-    double tot, rin;
-    if (block_sum_last2(d64, din, partial, partial2, &ps->ticket[3], &tot, &rin) && threadIdx.x == 0) {
+    block_partial(d64, part64);
+    block_partial(din, part_in);
+}
+
+// one block: totals of the residual partials and the start of the inner solve
+__global__ void __launch_bounds__(kCgThreads) k_residual_fin(const double* __restrict__ part64,
+                                                             const double* __restrict__ part_in, int np,
+                                                             PpeScalars* ps, double tol_in2, double floor2)
+{
+    const double tot = block_total(part64, np);
+    const double rin = block_total(part_in, np);
+    if (threadIdx.x == 0) {
         ps->rr64 = tot;
         ps->rin = rin;
         ps->rr = rin;
@@ -417,8 +427,10 @@ void launch_residual(int n, const int* rp, const int* col, const double* val, co
                      const double* x, V* e, V* r, V* p, double* partial, double* partial2,
                      PpeScalars* ps, double tol_in2, double floor2, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_residual<kG, V>), dim3(red_grid((long)n * kG)), dim3(256), 0, st, n, rp, col, val,
-                       b, x, e, r, p, partial, partial2, ps, tol_in2, floor2);
+    hipLaunchKernelGGL((k_residual<kG, V>), dim3(kCgBlocks), dim3(kCgThreads), 0, st, n, rp, col, val, b, x, e, r,
+                       p, partial, partial2);
+    hipLaunchKernelGGL(k_residual_fin, dim3(1), dim3(kCgThreads), 0, st, partial, partial2, kCgBlocks, ps, tol_in2,
+                       floor2);
 }
 
 template <typename V>

@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 EXE = os.path.join(os.path.dirname(mps.__file__), "mps_cal")
 
 
-@pytest.mark.parametrize("mode", [[], ["--semi"], ["--fp64"]])
+@pytest.mark.parametrize("mode", [[], ["--semi"], ["--fp64"], ["--vtk"]])
 def test_driver_runs_reference_files(tmp_path, mode):
     nb2, nb1, pos = small_dam(2)
     kw = {k: v for k, v in CFG0.items() if k != "g"}
@@ -25,8 +25,13 @@ def test_driver_runs_reference_files(tmp_path, mode):
     mps.geo_write(tmp_path / "GEO.dat", nb2, nb1, pos)
     r = subprocess.run([EXE, str(tmp_path)] + mode, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
-    outs = sorted(os.listdir(tmp_path / "out"))
+    outs = sorted(f for f in os.listdir(tmp_path / "out") if f.endswith(".out"))
     assert outs[0] == "0000.out" and len(outs) >= 2
+    vtks = sorted(f for f in os.listdir(tmp_path / "out") if f.endswith(".vtk"))
+    assert len(vtks) == (len(outs) if "--vtk" in mode else 0)
+    if vtks:
+        head = open(tmp_path / "out" / vtks[0]).read().split("\n")[:5]
+        assert head[0] == "# vtk DataFile Version 3.0" and head[4] == f"POINTS {len(pos)} float"
     tok = open(tmp_path / "out" / outs[-1]).read().split()
     assert len(tok) == 9 * len(pos)
     rec = np.array(tok, float).reshape(-1, 9)
