@@ -12,10 +12,15 @@ pressure, pressure gradient + update) over the resident state.
 
 Prints ONE JSON line: value = fluid particle-updates/s over the timed steps
 (inputs resident in HBM), roofline of the dominant kernel from HIP events on
-the library's stream, and the CPU oracle timed on this host on a bounded
-sample of the same workload.
+the library's stream (HBM, with the VALU issue rate beside it), the FP32 path
+against the FP64 path, the pressure Poisson solve (zero / previous-step /
+learned guess), and the CPU oracle timed on this host on a bounded sample of
+the same workload.
 
-    python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--no-cpu]
+    python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--no-cpu] [--no-ppe]
+    python bench.py --config 4 [--wavetank-nx 1000]   # semi-implicit wave tank line
+    python bench.py --ppe-only                          # the pressure Poisson part only
+    python bench.py --copy-calib 1024                   # 1 GiB device copy (counter scale)
 """
 from __future__ import annotations
 
@@ -147,7 +152,16 @@ def ppe_bench(mps, spin_steps=20, train_iters=2000, log=print):
                 best = {"ms": round(ms, 3), "iters": it, "relres": rel}
         res[name] = best
     s.close()
-    return {"workload": "dam break 0.52M fluid (64x64x128, l0=2.5mm), re_lap=3.1 l0, PPE "
+    # inner-iteration roofline of the plain solve: 8 nnz + 56 N algorithmic bytes per
+    # FP32 CG iteration (CSR values + columns, six N-vectors of 4 B, two FP64 partial sets)
+    pz = res["plain_zero"]
+    it_ms = pz["ms"] / max(pz["iters"], 1)
+    it_bytes = 8.0 * nnz + 56.0 * n
+    return {"cg_iteration": {"ms": it_ms, "alg_bytes": it_bytes,
+                             "achieved_GBs": it_bytes / (it_ms * 1e-3) / 1e9,
+                             "frac_hbm": it_bytes / (it_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                             "note": "solve ms / inner iterations: includes the FP64 residual passes"},
+            "workload": "dam break 0.52M fluid (64x64x128, l0=2.5mm), re_lap=3.1 l0, PPE "
                         f"(alpha = 1/(rho c^2) as MPS_CPU_EXPL), after {10 + spin_steps} semi-implicit "
                         "steps of dt=2.5e-4 s",
             "rows": n, "nnz": int(nnz), "solves": res,
@@ -156,6 +170,99 @@ def ppe_bench(mps, spin_steps=20, train_iters=2000, log=print):
             "train": {"samples": int(sum(len(x) for x in Xs)), "adam_steps": train_iters,
                       "loss_first": l0, "loss_last": l1, "train_s": round(train_s, 2),
                       "total_s": round(time.perf_counter() - t0, 2)}}
+
+
+# Config 4: wave tank 4.0 x 0.3 x 0.6 m inner at l0 = 4 mm (1000 x 75 x 150 sites),
+# water 0.3 m deep (5.625M fluid: the stated tank and depth at this l0), piston
+# x(t) = 0.05 sin(2 pi 0.8 t) on the x = 0 wall (13 sites of travel), 2 + 2 wall
+# layers, unperturbed lattice, semi-implicit step with the learned CG guess.
+WAVETANK = dict(name="wave tank 4.0x0.3x0.6 m inner, l0=4mm, water depth 0.3 m (1000x75x75 fluid), "
+                     "piston 0.05 sin(2 pi 0.8 t), 2+2 wall layers",
+                inner=(1000, 75, 150), depth=75, travel=13, dp=0.004, amp=0.05, freq=0.8)
+
+
+def wavetank_params(mps, dp, precision=0):
+    return mps.params(dim=3, precision=precision, rho=1000.0, nu=1e-6, dp=dp, re_n_mult=2.1,
+                      re_lap_mult=3.1, beta=0.97, cs=20.0, cfl=0.1, col_mult=0.8, col_rate=2.0,
+                      dt_max=4e-4, dt_min_mult=1e-3, max_dis=dp * 4, g=(0.0, 0.0, -9.81))
+
+
+def wavetank_bench(mps, steps, warmup, nx=None, log=print):
+    """Semi-implicit wave tank with the moving piston and the learned guess:
+    warm-up steps record (features, solution) pairs from the previous-pressure
+    solves, the network is trained on them, then `steps` timed steps run with
+    the learned guess.  The FP32 run is checked against the FP64 device path
+    over a short window from the same state."""
+    w = dict(WAVETANK)
+    if nx and nx != w["inner"][0]:
+        w["inner"] = (nx,) + w["inner"][1:]
+        w["name"] = w["name"].replace("4.0x0.3x0.6 m", f"{nx * w['dp']:.3g}x0.3x0.6 m").replace(
+            "(1000x75x75 fluid)", f"({nx}x75x75 fluid)")
+    nx, ny, nz = w["inner"]
+    nb2, nb1, pos, pist = mps.wavetank(nx, ny, nz, w["depth"], w["travel"], 2, 2, w["dp"], 0.0, 0)
+    N = len(pos)
+    nfl = N - nb2 - nb1
+    log(f"[bench] wave tank N={N} fluid={nfl}")
+    alpha = 1.0 / (1000.0 * 20.0 ** 2)
+    s = mps.Solver(wavetank_params(mps, w["dp"]), nb2, nb1, pos)
+    s.set_motion(pist, 0, w["amp"], w["freq"])
+    s.mlp_init(0)
+    s.mlp_set_record(True)
+    t0 = time.perf_counter()
+    it_w, ms_w = s.step_semi(warmup, alpha=alpha, x0_kind=1, tol=1e-8)
+    s.mlp_set_record(False)
+    l0, l1 = s.mlp_train(1000, batch=4096, lr=1e-3, seed=0)
+    log(f"[bench] {warmup} recorded steps ({it_w} CG iterations) and training in "
+        f"{time.perf_counter() - t0:.1f} s: loss {l0:.3e} -> {l1:.3e}")
+    t1 = time.perf_counter()
+    it, ms = s.step_semi(steps, alpha=alpha, x0_kind=3, tol=1e-8)
+    el = time.perf_counter() - t1
+    log(f"[bench] {steps} learned-guess steps in {el:.2f} s, {it} CG iterations")
+    # plain CG on the system of the next step, for the same-matrix comparison
+    s.stage_dt(); s.stage_sort(); s.stage_visc(); s.stage_collision(); s.stage_pnd()
+    nnz = s.ppe_assemble(alpha)
+    cmp = {}
+    for name, kind in (("plain_zero", 0), ("prev_step", 1), ("learned", 3)):
+        best = None
+        for rep in range(3):
+            t2 = time.perf_counter()
+            if kind == 3:
+                s.mlp_infer()
+            i2, rel = s.ppe_solve(mixed=True, x0_kind=kind, tol=1e-8)
+            ms2 = 1e3 * (time.perf_counter() - t2)
+            if best is None or ms2 < best["ms"]:
+                best = {"ms": round(ms2, 3), "iters": i2, "relres": rel}
+        cmp[name] = best
+    st32 = s.state(by_id=True)
+    sc = s.scalars()
+    s.close()
+    # FP64 device path over a short window from the initial state: error of p and u
+    k = 5
+    s32 = mps.Solver(wavetank_params(mps, w["dp"], 0), nb2, nb1, pos)
+    s64 = mps.Solver(wavetank_params(mps, w["dp"], 1), nb2, nb1, pos)
+    for q in (s32, s64):
+        q.set_motion(pist, 0, w["amp"], w["freq"])
+        q.step_semi(k, alpha=alpha, x0_kind=1, tol=1e-8)
+    a, b = s32.state(by_id=True), s64.state(by_id=True)
+    fl = a["type"] == 0
+    du = np.linalg.norm(a["vel"][fl] - b["vel"][fl], axis=1)
+    err = {"steps": k,
+           "p": float(np.max(np.abs(a["p"][fl] - b["p"][fl])) / max(np.max(b["p"][fl]), 1e-300)),
+           "p_max_Pa": float(np.max(b["p"][fl])),
+           "u": float(np.max(du) / max(np.max(np.linalg.norm(b["vel"][fl], axis=1)), 1e-300))}
+    s32.close()
+    s64.close()
+    return {"metric": "wave tank semi-implicit particle-updates/s; PPE ms/solve learned vs plain",
+            "value": nfl * steps / el, "unit": "particle-updates/s", "n_gpus": 1, "steps": steps,
+            "warmup": warmup, "ms_per_step": 1e3 * el / steps, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": w["name"], "n_total": N, "n_fluid": nfl, "dt_last": sc["dt"],
+                       "time": sc["time"], "re_lap": 3.1, "alpha": alpha},
+            "ppe": {"rows": N, "nnz": int(nnz), "solves_last_system": cmp,
+                    "timed_steps": {"cg_iterations": it, "solve_ms_total": ms,
+                                    "ms_per_solve": ms / max(steps, 1), "iters_per_solve": it / max(steps, 1)},
+                    "learned_over_plain_time": cmp["learned"]["ms"] / cmp["plain_zero"]["ms"]},
+            "fp64_error": err}
 
 
 def copy_calib(mib, reps=3):
@@ -204,10 +311,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--config", type=int, default=1, choices=sorted(WORKLOADS))
+    ap.add_argument("--config", type=int, default=1, choices=sorted(WORKLOADS) + [4],
+                    help="1 (default): explicit step at 1.05M fluid; 0: 16k fluid; 4: semi-implicit wave tank")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-ppe", action="store_true")
+    ap.add_argument("--wavetank-nx", type=int, default=1000,
+                    help="config 4: tank length in lattice sites (1000 = the 4.0 m tank)")
+    ap.add_argument("--ppe-only", action="store_true",
+                    help="only the pressure Poisson part (one JSON line with the ppe object)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter scaling) and exit")
     a = ap.parse_args()
@@ -221,6 +333,14 @@ def main():
 
     import mps_solvers_amd as mps
 
+    log = lambda m: print(m, file=sys.stderr, flush=True)
+    if a.ppe_only:
+        print(json.dumps({"ppe": ppe_bench(mps, log=log)}), flush=True)
+        return
+    if a.config == 4:
+        print(json.dumps(wavetank_bench(mps, a.steps if a.steps != 200 else 100, a.warmup, a.wavetank_nx,
+                                        log=log)), flush=True)
+        return
     w = WORKLOADS[a.config]
     nb2, nb1, pos = mps.dambreak(w["fluid"], w["inner"], 2, 2, w["dp"], 0.01, 0)
     N = len(pos)
@@ -252,6 +372,26 @@ def main():
         valu = {"instr_per_launch": vi, "achieved": va, "peak": VALU_PEAK_INSTR_S, "unit": "wave64 instr/s",
                 "frac": va / VALU_PEAK_INSTR_S}
     sc = s.scalars()
+    s.close()
+    # FP32 trajectory vs the FP64 device path (itself at ~1e-13 of the FP64 oracle in
+    # the parity tests) after the timed number of steps from the same initial state:
+    # max |p32 - p64| / max |p64| over fluid, the same for |u| as a vector, and the
+    # largest position difference
+    k = a.steps
+    a32 = mps.Solver(params_for(mps, w, 0), nb2, nb1, pos)
+    a64 = mps.Solver(params_for(mps, w, 1), nb2, nb1, pos)
+    a32.step(k)
+    a64.step(k)
+    g32, g64 = a32.state(by_id=True), a64.state(by_id=True)
+    a32.close()
+    a64.close()
+    flm = g32["type"] == 0
+    du = np.linalg.norm(g32["vel"][flm] - g64["vel"][flm], axis=1)
+    fp64_error = {"steps": k,
+                  "p": float(np.max(np.abs(g32["p"][flm] - g64["p"][flm])) / max(np.max(g64["p"][flm]), 1e-300)),
+                  "p_max_Pa": float(np.max(g64["p"][flm])),
+                  "u": float(np.max(du) / max(np.max(np.linalg.norm(g64["vel"][flm], axis=1)), 1e-300)),
+                  "x_max_abs_m": float(np.max(np.abs(g32["pos"] - g64["pos"])))}
     out = {
         "metric": "particle-updates/s at 1M particles; PPE ms/solve to 1e-8 residual, learned vs plain",
         "value": nfl * a.steps / el,
@@ -275,8 +415,8 @@ def main():
                      # the sweeps issue far more VALU work than their bytes need: the VALU
                      # issue rate is what bounds them (SQ_INSTS_VALU from the profile)
                      "valu": valu},
+        "fp64_error": fp64_error,
     }
-    s.close()
     if not a.no_ppe:
         print("[bench] pressure Poisson: plain vs previous-step vs learned guess", file=sys.stderr, flush=True)
         out["ppe"] = ppe_bench(mps, log=lambda m: print(m, file=sys.stderr, flush=True))

@@ -62,6 +62,7 @@ def lib():
         for f in ("orc_sim_destroy", "orc_sim_np"):
             getattr(_lib, f).argtypes = [C.c_void_p]
         _lib.orc_sim_step.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        _lib.orc_sim_step_semi.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double, I]
         _lib.orc_sim_get.argtypes = [C.c_void_p] + [D] * 8 + [I, I]
         _lib.orc_sim_consts.argtypes = [C.c_void_p, D]
         _lib.orc_sim_grid.argtypes = [C.c_void_p, C.POINTER(Grid)]
@@ -276,6 +277,16 @@ class Sim:
         rc = lib().orc_sim_step(self.h, nsteps, int(rebin_every_move))
         if rc:
             raise RuntimeError("dt below dt_min")
+
+    def step_semi(self, nsteps=1, alpha=0.0, tol=1e-8, rebin_every_move=False):
+        """Semi-implicit steps; returns the summed CG iterations."""
+        it = C.c_int()
+        rc = lib().orc_sim_step_semi(self.h, nsteps, int(rebin_every_move), float(alpha), float(tol), C.byref(it))
+        if rc == 1:
+            raise RuntimeError("dt below dt_min")
+        if rc == 2:
+            raise RuntimeError("CG did not converge")
+        return it.value
 
     def state(self):
         n = self.n

@@ -865,6 +865,69 @@ int orc_sim_step(orc_sim* S, int nsteps, int rebin_every_move)
     return 0;
 }
 
+/* MPS_CPU_EXPL::step (MPS_CAL_v4.0/MPS_CPU_EXPL.cpp:79-135) with two-pass
+ * operators, in the order of the device's semi-implicit step: dt, [re-bin],
+ * viscosity + g, collision, n*, buildPoisson (MPS_CPU.cpp:619-710), FP64 CG
+ * from the previous pressure to |b - A x| <= tol |b|, pressCorr p = max(x, 0)
+ * (MPS_CPU.cpp:734-743), pressure gradient without type-2 neighbours
+ * (MPS_CPU.cpp:384-445).  Returns 0, 1 when dt fell below dt_min, 2 when CG
+ * did not converge.  iters (may be NULL) receives the summed CG iterations. */
+int orc_sim_step_semi(orc_sim* S, int nsteps, int rebin_every_move, double alpha, double tol, int* iters)
+{
+    const int n = S->nt;
+    int* rp = (int*)malloc(sizeof(int) * ((size_t)n + 1));
+    double* b = (double*)malloc(sizeof(double) * (size_t)n);
+    double* x = (double*)malloc(sizeof(double) * (size_t)n);
+    unsigned char* bc = (unsigned char*)malloc((size_t)n);
+    int* col = NULL;
+    double* val = NULL;
+    long cap = 0;
+    int total = 0, rc = 0;
+    for (int s = 0; s < nsteps && !rc; s++) {
+        int small = 0;
+        S->dt = orc_update_dt(&S->P, S->np, S->u, S->v, S->w, &small);
+        if (small) { rc = 1; break; }
+        sim_motion(S, S->time + S->dt);
+        if (!rebin_every_move) sim_divide(S);
+        orc_visc(&S->P, &S->G, S->np, S->x, S->y, S->z, S->u, S->v, S->w, S->type, S->cs,
+                 S->coef, S->dt, S->x2, S->y2, S->z2, S->u2, S->v2, S->w2);
+        swap_state(S);
+        if (rebin_every_move) sim_divide(S);
+        orc_collision(&S->P, &S->G, S->np, S->x, S->y, S->z, S->u, S->v, S->w, S->type, S->cs,
+                      S->dt, S->x2, S->y2, S->z2, S->u2, S->v2, S->w2);
+        swap_state(S);
+        if (rebin_every_move) sim_divide(S);
+        orc_pnd(&S->P, &S->G, S->np, S->x, S->y, S->z, S->type, S->cs, S->n);
+        long nnz;
+        for (;;) {
+            nnz = orc_build_poisson(&S->P, &S->G, S->np, S->x, S->y, S->z, S->type, S->cs, S->n, S->p,
+                                    S->n0, S->lambda, S->dt, alpha, rp, col, val, b, x, bc, cap);
+            if (nnz >= 0 && nnz <= cap) break;
+            cap = (nnz < 0 ? -nnz : nnz) + 1024;
+            free(col); free(val);
+            col = (int*)malloc(sizeof(int) * (size_t)cap);
+            val = (double*)malloc(sizeof(double) * (size_t)cap);
+        }
+        const int it = orc_cg(S->np, rp, col, val, b, x, tol, 1, 100000, NULL);
+        if (it < 0) { rc = 2; break; }
+        total += it;
+        for (int i = 0; i < S->np; i++) S->p[i] = x[i] > 0.0 ? x[i] : 0.0;
+        if (S->P.f32)
+            for (int i = 0; i < S->np; i++) S->p[i] = (double)(float)S->p[i];
+        const int keep = S->P.gradp_skip_dummy;
+        S->P.gradp_skip_dummy = 1;
+        orc_gradp(&S->P, &S->G, S->np, S->x, S->y, S->z, S->u, S->v, S->w, S->p, S->type, S->cs,
+                  S->n0, S->dt, S->x2, S->y2, S->z2, S->u2, S->v2, S->w2);
+        S->P.gradp_skip_dummy = keep;
+        swap_state(S);
+        if (rebin_every_move) sim_divide(S);
+        S->time += S->dt;
+    }
+    if (iters) *iters = total;
+    free(rp); free(b); free(x); free(bc); free(col); free(val);
+    return rc;
+}
+
 int orc_sim_np(const orc_sim* S) { return S->np; }
 
 void orc_sim_get(const orc_sim* S, double* x, double* y, double* z, double* u, double* v,

@@ -101,6 +101,8 @@ orc_sim* orc_sim_create(const orc_params* P, int np, int nb2, int nb1, const dou
                         const double* w, const double* p);
 void orc_sim_destroy(orc_sim* S);
 int  orc_sim_step(orc_sim* S, int nsteps, int rebin_every_move);
+/* Semi-implicit steps (MPS_CPU_EXPL::step order, two-pass, FP64 CG to a relative tol). */
+int  orc_sim_step_semi(orc_sim* S, int nsteps, int rebin_every_move, double alpha, double tol, int* iters);
 int  orc_sim_np(const orc_sim* S);
 void orc_sim_get(const orc_sim* S, double* x, double* y, double* z, double* u, double* v,
                  double* w, double* p, double* n, int* type, int* id);

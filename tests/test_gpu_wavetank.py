@@ -108,3 +108,28 @@ def test_vtk_writer(tmp_path):
     assert np.max(np.abs(pts - st["pos"])) <= 5e-7
     assert np.max(np.abs(vel - st["vel"])) <= 5e-7
     assert np.max(np.abs(pr - st["p"])) <= 5e-7
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_semi_implicit_piston_matches_oracle(precision):
+    """The config-4 step (semi-implicit, piston moving) for 5 steps against the
+    oracle's semi-implicit simulator with the same motion; bars as the
+    semi-implicit trajectory test (p, u normwise 1e-5 FP64 / 2e-3 FP32)."""
+    nb2, nb1, pos, pist = _tank(jitter=0.0)
+    kw = dict(KW, re_lap_mult=3.1)
+    alpha = 1.0 / (1000.0 * 20.0 ** 2)
+    s = mps.Solver(gpu_params(precision, **kw), nb2, nb1, pos)
+    sim = oracle.Sim(oracle_params(precision, **kw), nb2, nb1, pos[:, 0], pos[:, 1], pos[:, 2])
+    s.set_motion(pist, 0, 0.05, 0.8)
+    sim.set_motion(pist, 0, 0.05, 0.8)
+    s.step_semi(5, alpha=alpha, x0_kind=1, tol=1e-8)
+    sim.step_semi(5, alpha=alpha, tol=1e-8)
+    g = s.state(by_id=True)
+    o = sim.state()
+    order = np.argsort(o["id"])
+    fl = g["type"] == 0
+    bar = 2e-3 if precision == 0 else 1e-5
+    assert np.max(o["p"]) > 0.0
+    assert normwise(g["p"][fl], o["p"][order][fl]) < bar
+    for a, k in enumerate("uvw"):
+        assert normwise(g["vel"][fl, a], o[k][order][fl]) < bar, k
