@@ -12,14 +12,19 @@
 // The solver replaces MPS_CPU::DoCG (MPS_CPU.cpp:829-888) and its cuBLAS twin
 // (MPS_CAL_v4.0/cuda_ker.cu:171-297): CG on the CSR matrix with FP32 SpMV
 // inside an FP64 iterative-refinement loop (or plain FP64 CG).  Reductions are
-// deterministic: per-block FP64 partials, summed in a fixed order by the last
-// block to finish.  No kernel waits on another block.
+// deterministic: per-block FP64 partials summed in a fixed order (in the CG
+// loop by every block of the consuming kernel, in the once-per-system row
+// setup by the last block to finish).  No kernel waits on another block.
+//
+// Assembly runs as two one-wavefront-per-cell sweeps over the staged 27-cell
+// candidates (mps_stage_wave.hpp): a count pass for the row lengths, a scan,
+// and a fill pass that writes each row's entries in ascending column order.
 #include <algorithm>
 
 #include "mps_device.hpp"
 #include "mps_kernels.hpp"
 #include "mps_ppe.hpp"
-#include "mps_tile.hpp"
+#include "mps_stage_wave.hpp"
 
 namespace mps {
 
@@ -52,17 +57,87 @@ __global__ void __launch_bounds__(256) k_ppe_rows(int n, StateView<Real> s, cons
 }
 
 // ------------------------------------------------------------- assembly ---
-// The 8 lanes of a row walk its candidate list with stride 8; in each round
-// the accepted entries of the 8 lanes are compacted in lane order, so the
-// row's columns are written in ascending order (the list ascends).
+// Row lengths for the CSR scan: one wavefront per cell, lane = row, each
+// staged candidate broadcast to all lanes; only the FP32 cutoff test (the
+// weights come in the fill pass).  A row counts its diagonal plus every j != i
+// with sqrt(rr) < RLAP, type_j != 2 and j not Dirichlet; identity rows count 1.
+constexpr int PPE_T2 = 1 << 30;   // candidate flag: type 2
+constexpr int PPE_BC = 1 << 29;   // candidate flag: Dirichlet
+constexpr int PPE_JM = PPE_BC - 1;
+
 template <typename Real>
-struct OpPpe {
+struct OpCount {
+    const Real *x, *y, *z;
+    const int* type;
+    const unsigned char* bcv;
+    Real tl;
+    int i, count;
+    bool active;
+    Real xi, yi, zi;
+
+    __device__ void load_cand(int j, wv::CandP<Real>& c)
+    {
+        c.x = x[j]; c.y = y[j]; c.z = z[j];
+        c.j = j | (type[j] == 2 ? PPE_T2 : 0) | (bcv[j] ? PPE_BC : 0);
+    }
+    __device__ void process(const wv::CandP<Real>* buf, int cnt)
+    {
+        for (int k = 0; k < cnt; k++) {
+            const wv::CandP<Real> c = buf[k];
+            const Real rr = rr3(c.x - xi, c.y - yi, c.z - zi);
+            count += ((rr < tl) & ((c.j & (PPE_T2 | PPE_BC)) == 0) & ((c.j & PPE_JM) != i)) ? 1 : 0;
+        }
+    }
+};
+
+template <typename Real, int CAP>
+__global__ void __launch_bounds__(64) k_ppe_count(Grid g, const int* __restrict__ cs, StateView<Real> s,
+                                                  Thresh<Real> th, const unsigned char* __restrict__ bc,
+                                                  int* __restrict__ row_ptr)
+{
+    __shared__ wv::CandP<Real> buf[CAP];
+    const int cell = xcd_cell(g.ncells);
+    if (cell >= g.ncells) return;
+    const int s0 = cs[cell], s1 = cs[cell + 1];
+    OpCount<Real> op;
+    op.x = s.x; op.y = s.y; op.z = s.z; op.type = s.type; op.bcv = bc;
+    op.tl = th.t_lap;
+    const Real R2 = wv::filt_r2(th.re_lap);
+    for (int c0 = s0; c0 < s1; c0 += WAVE) {
+        const int i = c0 + (int)threadIdx.x;
+        const bool in = i < s1;
+        op.i = i;
+        op.count = 0;
+        op.active = false;
+        if (in) {
+            op.xi = s.x[i]; op.yi = s.y[i]; op.zi = s.z[i];
+            op.active = !bc[i] && s.type[i] != 2;
+        }
+        Real lo[3], hi[3];
+        wv::wave_box(op.active, op.xi, op.yi, op.zi, lo, hi);
+        if (ballot(op.active)) wv::stage<CAP>(g, cs, cell, lo, hi, R2, buf, op);
+        if (in) row_ptr[i] = op.active ? op.count + 1 : 1;
+        __syncthreads();
+    }
+}
+
+// Fill pass, two phases per LDS batch as in the PND sweep: the FP32 cutoff
+// test appends the row's diagonal and in-range non-dummy candidates to a
+// per-lane list (slot-major in LDS), then the FP64 weights run over the dense
+// lists and each lane writes its row's entries at consecutive CSR slots, in
+// the staged (ascending) order.  Dirichlet neighbours add to the diagonal but
+// get no entry.  The diagonal's value is written when the row is complete.
+constexpr int FILL_LMAX = 32;
+constexpr uint16_t FILL_DIAG = 0x8000u;
+
+template <typename Real>
+struct OpFill {
     const Real *x, *y, *z;
     const int* type;
     const unsigned char* bcv;
     Real tl;
     double re;
-    int i, sub, fill;
+    int i;
     bool active;
     Real xi, yi, zi;
     double diag;
@@ -70,107 +145,106 @@ struct OpPpe {
     int* col;
     float* v32;
     double* v64;
+    uint16_t* L;   // L[s * WAVE + lane]
 
-    __device__ void load_cand(int j, Cand<Real>& c)
+    __device__ void load_cand(int j, wv::CandP<Real>& c)
     {
-        c.x = x[j]; c.y = y[j]; c.z = z[j]; c.j = j;
-        c.t = type[j] | (bcv[j] ? 16 : 0);
+        c.x = x[j]; c.y = y[j]; c.z = z[j];
+        c.j = j | (type[j] == 2 ? PPE_T2 : 0) | (bcv[j] ? PPE_BC : 0);
     }
-    __device__ void process(const Cand<Real>* S, const uint16_t* L, int len)
+    __device__ void drain(const wv::CandP<Real>* buf, int m)
     {
-        const int lane = threadIdx.x & 63;
-        for (int k0 = 0; k0 < len; k0 += TK) {
-            const int k = k0 + sub;
-            bool acc = false, isdiag = false;
-            double w = 0.0;
-            int cj = -1;
-            if (active && k < len) {
-                const Cand<Real>& c = S[L[k]];
-                cj = c.j;
-                if (cj == i) {
-                    isdiag = true;
-                    acc = true;
-                } else if ((c.t & 15) != 2) {
-                    const Real rr = rr3(c.x - xi, c.y - yi, c.z - zi);
-                    if (rr < tl) {
-                        const double dx = (double)c.x - (double)xi, dy = (double)c.y - (double)yi,
-                                     dz = (double)c.z - (double)zi;
-                        w = __builtin_fma(re, rsq(dx * dx + dy * dy + dz * dz), -1.0);
-                        diag -= w;
-                        acc = !(c.t & 16);   // Dirichlet neighbour: A_ij = 0
+        const int lane = threadIdx.x;
+        const int mm = wave_max(m);
+        const double xd = (double)xi, yd = (double)yi, zd = (double)zi;
+        for (int s = 0; s < mm; s++) {
+            if (s < m) {
+                const uint16_t lk = L[s * WAVE + lane];
+                if (lk & FILL_DIAG) {
+                    kdiag = kpos;
+                    col[kpos++] = i;
+                } else {
+                    const wv::CandP<Real> c = buf[lk];
+                    const double dx = (double)c.x - xd, dy = (double)c.y - yd, dz = (double)c.z - zd;
+                    const double w = __builtin_fma(re, rsq(dx * dx + dy * dy + dz * dz), -1.0);
+                    diag -= w;
+                    if (!(c.j & PPE_BC)) {   // Dirichlet neighbour: A_ij = 0
+                        col[kpos] = c.j & PPE_JM;
+                        v64[kpos] = w;
+                        v32[kpos] = (float)w;
+                        kpos++;
                     }
                 }
             }
-            const uint64_t m = ballot(acc);
-            const uint32_t gm = (uint32_t)(m >> (lane & ~7)) & 0xFFu;
-            const int pre = __popc(gm & ((1u << sub) - 1u));
-            if (acc && fill) {
-                const long pos = kpos + pre;
-                col[pos] = cj;
-                if (isdiag) kdiag = pos;
-                else { v64[pos] = w; v32[pos] = (float)w; }
-            }
-            kpos += __popc(gm);
         }
+    }
+    __device__ void process(const wv::CandP<Real>* buf, int cnt)
+    {
+        int m = 0;
+        for (int k = 0; k < cnt; k++) {
+            const wv::CandP<Real> c = buf[k];
+            const Real rr = rr3(c.x - xi, c.y - yi, c.z - zi);
+            const bool self = (c.j & PPE_JM) == i;
+            if (active && (self || ((rr < tl) & !(c.j & PPE_T2)))) {
+                L[m * WAVE + threadIdx.x] = (uint16_t)(self ? FILL_DIAG : k);
+                m++;
+            }
+            // at most 16 appends between checks: no lane passes FILL_LMAX
+            if ((k & 15) == 15 && ballot(m > FILL_LMAX - 16)) {
+                drain(buf, m);
+                m = 0;
+            }
+        }
+        drain(buf, m);
     }
 };
 
-template <typename Real>
-__global__ void __launch_bounds__(TB) k_ppe_assemble(Grid g, const int* __restrict__ cs, StateView<Real> s,
-                                                     Thresh<Real> th, const unsigned char* __restrict__ bc,
-                                                     const DevScalars* __restrict__ sc, double aii_over,
-                                                     int fill, int* __restrict__ row_ptr,
-                                                     int* __restrict__ col, float* __restrict__ v32,
-                                                     double* __restrict__ v64)
+template <typename Real, int CAP>
+__global__ void __launch_bounds__(64) k_ppe_fill(Grid g, const int* __restrict__ cs, StateView<Real> s,
+                                                 Thresh<Real> th, const unsigned char* __restrict__ bc,
+                                                 const DevScalars* __restrict__ sc, double aii_over,
+                                                 const int* __restrict__ row_ptr, int* __restrict__ col,
+                                                 float* __restrict__ v32, double* __restrict__ v64)
 {
-    __shared__ TileShared<Real> sh;
+    __shared__ wv::CandP<Real> buf[CAP];
+    __shared__ uint16_t L[FILL_LMAX * WAVE];
     const int cell = xcd_cell(g.ncells);
     if (cell >= g.ncells) return;
     const int s0 = cs[cell], s1 = cs[cell + 1];
     // rho lambda alpha n0 / (2 d dt^2)
     const double dt = sc->dt;
     const double aii = aii_over * sc->lambda * sc->n0 / (dt * dt);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, sub = lane & 7;
-    OpPpe<Real> op;
+    OpFill<Real> op;
     op.x = s.x; op.y = s.y; op.z = s.z; op.type = s.type; op.bcv = bc;
-    op.tl = th.t_lap; op.re = (double)th.re_lap; op.fill = fill; op.sub = sub;
-    op.col = col; op.v32 = v32; op.v64 = v64;
-    const Real R2 = filt_r2(th.re_lap);
-    for (int c0 = s0; c0 < s1; c0 += TW * 8) {
-        const int i = c0 + wave * 8 + (lane >> 3);
+    op.tl = th.t_lap; op.re = (double)th.re_lap;
+    op.col = col; op.v32 = v32; op.v64 = v64; op.L = L;
+    const Real R2 = wv::filt_r2(th.re_lap);
+    for (int c0 = s0; c0 < s1; c0 += WAVE) {
+        const int i = c0 + (int)threadIdx.x;
         const bool in = i < s1;
         op.i = i;
         op.active = false;
         op.diag = 0.0;
-        op.kpos = fill && in ? row_ptr[i] : 0;
+        op.kpos = in ? row_ptr[i] : 0;
         op.kdiag = -1;
-        op.xi = op.yi = op.zi = (Real)0;
         if (in) {
             op.xi = s.x[i]; op.yi = s.y[i]; op.zi = s.z[i];
             op.active = !bc[i] && s.type[i] != 2;
         }
-        Real wlo[3], whi[3], blo[3], bhi[3];
-        bool tact;
-        if (tile_boxes(sh, op.active, op.xi, op.yi, op.zi, wlo, whi, blo, bhi, tact))
-            tile_stage(g, cs, cell, sh, blo, bhi, wlo, whi, R2, tact, op);
-        const double diag = sub_sum(op.diag);
-        const long kdiag = sub_max(op.kdiag);
-        if (in && sub == 0) {
+        Real lo[3], hi[3];
+        wv::wave_box(op.active, op.xi, op.yi, op.zi, lo, hi);
+        if (ballot(op.active)) wv::stage<CAP>(g, cs, cell, lo, hi, R2, buf, op);
+        if (in) {
             if (!op.active) {
-                if (fill) {
-                    const long k = row_ptr[i];
-                    col[k] = i; v64[k] = 1.0; v32[k] = 1.0f;
-                } else {
-                    row_ptr[i] = 1;
-                }
-            } else if (fill) {
-                const double d = diag - aii;
-                v64[kdiag] = d;
-                v32[kdiag] = (float)d;
-            } else {
-                row_ptr[i] = (int)op.kpos;
+                const long k = op.kpos;   // identity row
+                col[k] = i; v64[k] = 1.0; v32[k] = 1.0f;
+            } else if (op.kdiag >= 0) {   // the row's own particle is always staged
+                const double d = op.diag - aii;
+                v64[op.kdiag] = d;
+                v32[op.kdiag] = (float)d;
             }
         }
+        __syncthreads();
     }
 }
 
@@ -407,8 +481,13 @@ void launch_ppe_assemble(const Grid& g, const int* cs, StateView<Real> s, const 
                          const unsigned char* bc, const DevScalars* sc, double aii_over, int fill,
                          int* row_ptr, int* col, float* v32, double* v64, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_ppe_assemble<Real>, dim3(xcd_grid(g.ncells)), dim3(TB), 0, st, g, cs, s, th, bc,
-                       sc, aii_over, fill, row_ptr, col, v32, v64);
+    if (!fill) {
+        hipLaunchKernelGGL((k_ppe_count<Real, wv::cap_of<Real>(128)>), dim3(xcd_grid(g.ncells)), dim3(WAVE), 0, st,
+                           g, cs, s, th, bc, row_ptr);
+        return;
+    }
+    hipLaunchKernelGGL((k_ppe_fill<Real, wv::cap_of<Real>(128)>), dim3(xcd_grid(g.ncells)), dim3(WAVE), 0, st, g,
+                       cs, s, th, bc, sc, aii_over, row_ptr, col, v32, v64);
 }
 
 template <typename MV, typename V>
