@@ -539,17 +539,6 @@ extern "C" int mps_create(const mps_params* p, int np, int nb2, int nb1, const d
     k.g[0] = p->g[0]; k.g[1] = p->g[1]; k.g[2] = p->g[2];
     k.dim = p->dim;
     k.gradp_skip_dummy = p->gradp_skip_dummy;
-    {
-        // sweep kernels: one wavefront per cell by default; MPS_SWEEP=tile selects
-        // the eight-lanes-per-particle variant
-        const char* v = getenv("MPS_SWEEP");
-        k.variant = (v && !strcmp(v, "tile")) ? 1 : 0;
-        // LDS batch of the one-wavefront sweeps (float candidates): 128 by
-        // default (4 KiB: the most resident wavefronts per CU); MPS_WAVE_CAP=256|512
-        const char* wc = getenv("MPS_WAVE_CAP");
-        const int cap = wc ? atoi(wc) : 128;
-        k.wave_cap = (cap == 256 || cap == 512) ? cap : 128;
-    }
     double fl2 = 0.0, st2 = 0.0;
     for (int i = 0; i < np; i++) {
         double a = vel ? vel[3 * i] * vel[3 * i] + vel[3 * i + 1] * vel[3 * i + 1] + vel[3 * i + 2] * vel[3 * i + 2] : 0.0;

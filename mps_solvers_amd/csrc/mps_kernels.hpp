@@ -37,8 +37,7 @@ struct StepConst {
     double mov_amp, mov_omega;   // prescribed motion (amplitude 0: none); its speed enters dt
     int dim;
     int gradp_skip_dummy;
-    int variant;   // neighbour-sweep kernels: 0 one wavefront per cell, 1 eight-particle tiles
-    int wave_cap;  // one-wavefront sweeps: LDS batch in float candidates (128, 256 or 512)
+    int pad_;
 };
 
 namespace wv {

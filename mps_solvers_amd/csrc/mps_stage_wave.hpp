@@ -1,5 +1,5 @@
 // mps_stage_wave.hpp -- one-wavefront-per-cell LDS staging of the 27-cell
-// candidate set (the "wave" sweep variant).
+// candidate set, shared by the neighbour sweeps and the Poisson assembly.
 #pragma once
 
 #include <float.h>
@@ -25,8 +25,8 @@ struct alignas(16) CandP {
     int j;
 };
 
-// LDS batch capacity in candidates: `cap` is the float capacity chosen per
-// context (256 or 512); FP64 candidates are twice as large, so half as many.
+// LDS batch capacity in candidates for a float capacity `cap`: FP64
+// candidates are twice as large, so half as many.
 template <typename Real> __host__ __device__ constexpr int cap_of(int cap) { return sizeof(Real) == 4 ? cap : cap / 2; }
 
 template <typename Real> __device__ __forceinline__ Real big();

@@ -1,9 +1,7 @@
 // mps_sweeps_wave.hip -- the one-wavefront-per-cell neighbour sweeps: one
 // 64-lane wavefront per cell (chunks of 64 particles, lane = particle), the
 // 27-cell candidate set filtered against the wave's bounding box and staged
-// through LDS in batches, each staged candidate broadcast to all lanes
-// (StepConst::variant 0, the default).  Same operator semantics as
-// mps_sweeps.hip (variant 1, eight lanes per particle).
+// through LDS in batches, each staged candidate broadcast to all lanes.
 #include <float.h>
 
 #include "mps_device.hpp"
@@ -386,20 +384,13 @@ __global__ void __launch_bounds__(64) k_gradp(Grid g, const int* __restrict__ cs
 }
 
 
-// k.wave_cap selects the LDS batch (128, 256 or 512 float candidates): the smaller
-// batch doubles the resident wavefronts per CU at the price of more batches.
+// LDS batches of 128 FP32 candidates (4 KiB; 64 FP64 candidates): VGPRs, not
+// LDS, then bound the resident wavefronts (8 per SIMD).  256- and 512-candidate
+// batches measured slower (fewer resident wavefronts).
+constexpr int WAVE_CAP = 128;
 #define MPS_WAVE_LAUNCH(kern, ...)                                                                          \
-    do {                                                                                                   \
-        if (k.wave_cap == 512)                                                                             \
-            hipLaunchKernelGGL((kern<Real, cap_of<Real>(512)>), dim3(xcd_grid(g.ncells)), dim3(64), 0, st, \
-                               __VA_ARGS__);                                                               \
-        else if (k.wave_cap == 256)                                                                        \
-            hipLaunchKernelGGL((kern<Real, cap_of<Real>(256)>), dim3(xcd_grid(g.ncells)), dim3(64), 0, st, \
-                               __VA_ARGS__);                                                               \
-        else                                                                                               \
-            hipLaunchKernelGGL((kern<Real, cap_of<Real>(128)>), dim3(xcd_grid(g.ncells)), dim3(64), 0, st, \
-                               __VA_ARGS__);                                                               \
-    } while (0)
+    hipLaunchKernelGGL((kern<Real, cap_of<Real>(WAVE_CAP)>), dim3(xcd_grid(g.ncells)), dim3(64), 0, st, \
+                       __VA_ARGS__)
 
 template <typename Real>
 void launch_pnd(const Grid& g, const int* cs, StateView<Real> s, const Thresh<Real>& th, const StepConst& k,
