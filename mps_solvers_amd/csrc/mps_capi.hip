@@ -370,6 +370,7 @@ static int divide_t(mps_ctx* c)
     launch_radix_sort(c->nt, c->key_bits, c->k0, c->v0, c->k1, c->v1, c->hist, &kr, &vr, c->st);
     launch_cell_bounds(kr, c->nt, c->g.ncells, c->cs, c->st);
     launch_reorder<Real>(c->nt, vr, cur, dst, c->inv, c->ntmp, c->st);
+    launch_tail_copy<Real>(c->nt, c->cs + c->g.ncells, dst, cur, c->st);
     c->perm = vr;
     c->kin ^= 1;
     c->aux ^= 1;

@@ -66,6 +66,10 @@ void launch_cell_bounds(const uint32_t* keys, int n, int ncells, int* cs, hipStr
 template <typename Real>
 void launch_reorder(int n, const uint32_t* perm, StateView<Real> src, StateView<Real> dst, int* inv,
                     int* normal_tmp, hipStream_t st);
+// copy positions and velocities of the excluded slots [cs[ncells], n) from one
+// state buffer into the other, so that both hold them until the next sort
+template <typename Real>
+void launch_tail_copy(int n, const int* np_active, StateView<Real> from, StateView<Real> to, hipStream_t st);
 
 // sweeps
 template <typename Real>

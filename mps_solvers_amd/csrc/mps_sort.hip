@@ -244,6 +244,19 @@ __global__ void __launch_bounds__(256) k_normal_remap(int n, const int* __restri
     normal[k] = o >= 0 ? inv[o] : -1;
 }
 
+// Excluded particles sit after every cell; the sweeps write cells only, so the
+// tail of the ping-pong buffer they write into would hold another buffer's
+// stale values.  After each sort both buffers get the sorted tail.
+template <typename Real>
+__global__ void __launch_bounds__(256) k_tail_copy(int n, const int* __restrict__ np_active,
+                                                   StateView<Real> from, StateView<Real> to)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n || k < *np_active) return;
+    to.x[k] = from.x[k]; to.y[k] = from.y[k]; to.z[k] = from.z[k];
+    to.u[k] = from.u[k]; to.v[k] = from.v[k]; to.w[k] = from.w[k];
+}
+
 // ------------------------------------------------------------- drivers ---
 static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
@@ -306,6 +319,15 @@ void launch_reorder(int n, const uint32_t* perm, StateView<Real> src, StateView<
                        dst.normal);
 }
 
+template <typename Real>
+void launch_tail_copy(int n, const int* np_active, StateView<Real> from, StateView<Real> to, hipStream_t st)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_tail_copy<Real>, dim3(cdiv(n, 256)), dim3(256), 0, st, n, np_active, from, to);
+}
+
+template void launch_tail_copy<float>(int, const int*, StateView<float>, StateView<float>, hipStream_t);
+template void launch_tail_copy<double>(int, const int*, StateView<double>, StateView<double>, hipStream_t);
 template void launch_hash<float>(const Grid&, int, StateView<float>, uint32_t*, uint32_t*, int*, hipStream_t);
 template void launch_hash<double>(const Grid&, int, StateView<double>, uint32_t*, uint32_t*, int*, hipStream_t);
 template void launch_reorder<float>(int, const uint32_t*, StateView<float>, StateView<float>, int*, int*, hipStream_t);

@@ -777,8 +777,21 @@ void orc_sim_destroy(orc_sim* S)
     free(S);
 }
 
+/* The operators write the np particles inside cells; the excluded tail
+ * [np, nt) is carried into the new buffer so that an excluded particle keeps
+ * the position it left the grid at (the reference overwrites slot np-1
+ * instead, MPS_GPU.cpp:311-314). */
 static void swap_state(orc_sim* S)
 {
+    const size_t tail = sizeof(double) * (size_t)(S->nt - S->np);
+    if (tail) {
+        memcpy(S->x2 + S->np, S->x + S->np, tail);
+        memcpy(S->y2 + S->np, S->y + S->np, tail);
+        memcpy(S->z2 + S->np, S->z + S->np, tail);
+        memcpy(S->u2 + S->np, S->u + S->np, tail);
+        memcpy(S->v2 + S->np, S->v + S->np, tail);
+        memcpy(S->w2 + S->np, S->w + S->np, tail);
+    }
     double* t;
     t = S->x; S->x = S->x2; S->x2 = t;
     t = S->y; S->y = S->y2; S->y2 = t;

@@ -40,6 +40,26 @@ def test_create_without_device_fails_loudly():
     assert e.value.code == mps.MPS_ERR_NODEV
 
 
+def test_create_rejects_bad_arguments_before_touching_a_device():
+    """Empty, inconsistent or wrongly dimensioned scenes are MPS_ERR_ARG (the
+    reference would exit or read out of bounds); checked before any device call."""
+    lib = mps.lib()
+    P = mps.params()
+    pos = np.zeros((4, 3))
+    D = ctypes.POINTER(ctypes.c_double)
+    h = ctypes.c_void_p()
+    pp = pos.ctypes.data_as(D)
+    assert lib.mps_create(ctypes.byref(P), 0, 0, 0, pp, None, None, ctypes.byref(h)) == mps.MPS_ERR_ARG
+    assert lib.mps_create(ctypes.byref(P), 4, 3, 2, pp, None, None, ctypes.byref(h)) == mps.MPS_ERR_ARG
+    assert lib.mps_create(ctypes.byref(P), 4, 0, 0, None, None, None, ctypes.byref(h)) == mps.MPS_ERR_ARG
+    P.dim = 4
+    assert lib.mps_create(ctypes.byref(P), 4, 0, 0, pp, None, None, ctypes.byref(h)) == mps.MPS_ERR_ARG
+    n, a, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    assert lib.mps_scene_wavetank(10, 4, 8, 9, 2, 2, 2, 0.004, 0.0, 0, ctypes.byref(n), ctypes.byref(a),
+                                  ctypes.byref(b), None, None) == mps.MPS_ERR_ARG   # depth > nz
+    assert lib.mps_set_motion(None, None, 0, 0.0, 0.0) == mps.MPS_ERR_ARG
+
+
 def _np_dambreak(fluid, inner, lw, ld, dp, jitter, seed, dim):
     """numpy restatement of mps_scene_dambreak (order dummy, wall, fluid; x fastest)."""
     L = lw + ld
