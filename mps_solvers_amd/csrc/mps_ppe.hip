@@ -82,11 +82,10 @@ struct OpCount {
     }
     __device__ void process(const wv::CandP<Real>* buf, int cnt)
     {
-        for (int k = 0; k < cnt; k++) {
-            const wv::CandP<Real> c = buf[k];
+        wv::for_each_cand<8>(buf, cnt, [&](const wv::CandP<Real>& c, int) {
             const Real rr = rr3(c.x - xi, c.y - yi, c.z - zi);
             count += ((rr < tl) & ((c.j & (PPE_T2 | PPE_BC)) == 0) & ((c.j & PPE_JM) != i)) ? 1 : 0;
-        }
+        });
     }
 };
 
@@ -181,16 +180,17 @@ struct OpFill {
     __device__ void process(const wv::CandP<Real>* buf, int cnt)
     {
         int m = 0;
-        for (int k = 0; k < cnt; k++) {
-            const wv::CandP<Real> c = buf[k];
-            const Real rr = rr3(c.x - xi, c.y - yi, c.z - zi);
-            const bool self = (c.j & PPE_JM) == i;
-            if (active && (self || ((rr < tl) & !(c.j & PPE_T2)))) {
-                L[m * WAVE + threadIdx.x] = (uint16_t)(self ? FILL_DIAG : k);
-                m++;
-            }
+        for (int k0 = 0; k0 < cnt; k0 += 16) {
+            wv::for_each_cand<8>(buf + k0, min(16, cnt - k0), [&](const wv::CandP<Real>& c, int k) {
+                const Real rr = rr3(c.x - xi, c.y - yi, c.z - zi);
+                const bool self = (c.j & PPE_JM) == i;
+                if (active && (self || ((rr < tl) & !(c.j & PPE_T2)))) {
+                    L[m * WAVE + threadIdx.x] = (uint16_t)(self ? FILL_DIAG : k0 + k);
+                    m++;
+                }
+            });
             // at most 16 appends between checks: no lane passes FILL_LMAX
-            if ((k & 15) == 15 && ballot(m > FILL_LMAX - 16)) {
+            if (ballot(m > FILL_LMAX - 16)) {
                 drain(buf, m);
                 m = 0;
             }
@@ -250,28 +250,37 @@ __global__ void __launch_bounds__(64) k_ppe_fill(Grid g, const int* __restrict__
 
 // ----------------------------------------------------------------- SpMV ---
 // CSR-vector: G lanes per row, lane-strided partial sums, then a shuffle tree
-// over the G lanes (fixed order: deterministic).  Each lane issues its column
-// and value loads four entries at a time before the gathers of p, so that
-// several loads are in flight per lane; the lane still adds its entries in
-// ascending order.
-template <int G, typename MV, typename V, typename Acc>
+// over the G lanes (fixed order: deterministic).  A lane loads its next U
+// column/value pairs in one predicated batch (slots past the row end read
+// nothing and add an exact zero), then issues the U gathers of p together, so
+// a row of up to G*U entries costs two dependent memory latencies.  The lane
+// still adds its entries in ascending order.
+// batch depth of row_dot: with 16 lanes a row, 6 slots cover the ~93 entries
+// of an re = 3.1 l0 row in one batch (measured best of 4/6/8; non-temporal
+// matrix loads were slower)
+constexpr int kU = 6;
+template <int G, int U, typename MV, typename V, typename Acc>
 __device__ __forceinline__ Acc row_dot(int row, int g, const int* __restrict__ rp,
                                        const int* __restrict__ col, const MV* __restrict__ val,
                                        const V* __restrict__ p)
 {
     Acc a = (Acc)0;
     const int k1 = rp[row + 1];
-    int k = rp[row] + g;
-    for (; k + 3 * G < k1; k += 4 * G) {
-        const int c0 = col[k], c1 = col[k + G], c2 = col[k + 2 * G], c3 = col[k + 3 * G];
-        const MV v0 = val[k], v1 = val[k + G], v2 = val[k + 2 * G], v3 = val[k + 3 * G];
-        const V p0 = p[c0], p1 = p[c1], p2 = p[c2], p3 = p[c3];
-        a += (Acc)v0 * (Acc)p0;
-        a += (Acc)v1 * (Acc)p1;
-        a += (Acc)v2 * (Acc)p2;
-        a += (Acc)v3 * (Acc)p3;
+    for (int k = rp[row] + g; k < k1; k += U * G) {
+        int c[U];
+        MV v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int kk = k + u * G;
+            c[u] = kk < k1 ? col[kk] : -1;
+            v[u] = kk < k1 ? val[kk] : (MV)0;
+        }
+        V pv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) pv[u] = c[u] >= 0 ? p[c[u]] : (V)0;
+#pragma unroll
+        for (int u = 0; u < U; u++) a += (Acc)v[u] * (Acc)pv[u];
     }
-    for (; k < k1; k += G) a += (Acc)val[k] * (Acc)p[col[k]];
 #pragma unroll
     for (int m = G / 2; m >= 1; m >>= 1) a += __shfl_xor(a, m);
     return a;
@@ -330,7 +339,7 @@ __global__ void __launch_bounds__(kCgThreads) k_spmv_dot(int n, const int* __res
     const int g = threadIdx.x % G, rpb = kCgThreads / G;
     double d = 0.0;
     for (int row = blockIdx.x * rpb + threadIdx.x / G; row < n; row += gridDim.x * rpb) {
-        const V a = row_dot<G, MV, V, V>(row, g, rp, col, val, p);
+        const V a = row_dot<G, kU, MV, V, V>(row, g, rp, col, val, p);
         if (g == 0) {
             q[row] = a;
             d += (double)p[row] * (double)a;
@@ -405,7 +414,7 @@ __global__ void __launch_bounds__(kCgThreads) k_residual(int n, const int* __res
     const int g = threadIdx.x % G, rpb = kCgThreads / G;
     double d64 = 0.0, din = 0.0;
     for (int row = blockIdx.x * rpb + threadIdx.x / G; row < n; row += gridDim.x * rpb) {
-        const double a = row_dot<G, double, double, double>(row, g, rp, col, val, x);
+        const double a = row_dot<G, kU, double, double, double>(row, g, rp, col, val, x);
         if (g == 0) {
             const double ri = b[row] - a;
             const V rv = (V)ri;

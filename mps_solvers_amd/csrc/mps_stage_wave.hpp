@@ -44,6 +44,26 @@ __device__ __forceinline__ Real box_d2(Real qx, Real qy, Real qz, const Real* lo
 
 struct StageResult { int count; bool overflowed; };
 
+// Visit buf[0..cnt) in ascending order.  Candidates are read from LDS U at a
+// time before the first of them is visited, so that the LDS latency of a
+// group overlaps the arithmetic of the group before it (a plain loop waits
+// on every read).  U trades that overlap against registers: each sweep uses
+// the U measured best for it (a larger U that costs a resident wavefront per
+// SIMD loses more than it hides).
+template <int U, class Cd, class F>
+__device__ __forceinline__ void for_each_cand(const Cd* buf, int cnt, F&& f)
+{
+    int k = 0;
+    for (; k + U <= cnt; k += U) {
+        Cd c[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) c[u] = buf[k + u];
+#pragma unroll
+        for (int u = 0; u < U; u++) f(c[u], k + u);
+    }
+    for (; k < cnt; k++) f(buf[k], k);
+}
+
 // Stream the stencil of `cell`, keep candidates within sqrt(R2) of the box,
 // and hand full LDS batches (CAP candidates of type Cd) to op.process().  The
 // last batch stays in `buf`.

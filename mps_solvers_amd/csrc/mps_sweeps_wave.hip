@@ -68,10 +68,10 @@ struct OpPnd {
     __device__ void process(const CandP<Real>* buf, int cnt)
     {
         int m = 0;
-        for (int k = 0; k < cnt; k++) {
-            test(buf[k], k, m);
+        for (int k0 = 0; k0 < cnt; k0 += 16) {
+            for_each_cand<8>(buf + k0, min(16, cnt - k0), [&](const CandP<Real>& c, int k) { test(c, k0 + k, m); });
             // at most 16 appends between checks: no lane passes PND_LMAX
-            if ((k & 15) == 15 && ballot(m > PND_LMAX - 16)) {
+            if (ballot(m > PND_LMAX - 16)) {
                 drain(buf, m);
                 m = 0;
             }
@@ -142,7 +142,7 @@ struct OpVisc {
     }
     __device__ void process(const Cand<Real>* buf, int cnt)
     {
-        for (int k = 0; k < cnt; k++) visit(buf[k]);
+        for_each_cand<1>(buf, cnt, [&](const Cand<Real>& c, int) { visit(c); });
     }
 };
 
@@ -228,7 +228,7 @@ struct OpCol {
     }
     __device__ void process(const Cand<Real>* buf, int cnt)
     {
-        for (int k = 0; k < cnt; k++) visit(buf[k]);
+        for_each_cand<1>(buf, cnt, [&](const Cand<Real>& c, int) { visit(c); });
     }
 };
 
@@ -315,9 +315,9 @@ struct OpGrad {
     __device__ void process(const Cand<Real>* buf, int cnt)
     {
         if (pass == 0) {
-            for (int k = 0; k < cnt; k++) visit_min(buf[k]);
+            for_each_cand<4>(buf, cnt, [&](const Cand<Real>& c, int) { visit_min(c); });
         } else {
-            for (int k = 0; k < cnt; k++) visit_grad(buf[k]);
+            for_each_cand<4>(buf, cnt, [&](const Cand<Real>& c, int) { visit_grad(c); });
         }
     }
 };
