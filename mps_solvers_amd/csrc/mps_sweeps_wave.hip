@@ -279,46 +279,47 @@ __global__ void __launch_bounds__(64) k_collision(Grid g, const int* __restrict_
 }
 
 // ------------------------------------------------------ pressure gradient ---
+// One pass over the candidates.  The reference's sum (MPS_GPU.cpp:571-638)
+//   sum_j (p_j - p_hat) w_ij / r_ij^2 r_ij,  p_hat = min(p_i, p_j : type_j != 2, r^2 <= R0^2)
+// needs p_hat before it starts; it equals
+//   sum_j (p_j - p_i) w_ij / r_ij^2 r_ij  +  (p_i - p_hat) sum_j w_ij / r_ij^2 r_ij,
+// so both sums and p_hat accumulate together and combine at the end.  The
+// terms are the natural differences p_j - p_i; no cancellation is added.
 template <typename Real>
 struct OpGrad {
     const Real *x, *y, *z, *p;
     const int* type;
     Real re, tn, re2;
-    int i, pass, skip_dummy;
+    int i, skip_dummy;
     bool active;
-    Real xi, yi, zi, ph;
-    Real gx, gy, gz;
+    Real xi, yi, zi, pi, ph;
+    Real ax, ay, az, bx, by, bz;
 
     __device__ void load_cand(int j, Cand<Real>& c)
     {
         c.x = x[j]; c.y = y[j]; c.z = z[j]; c.j = j;
         c.t = type[j]; c.a = p[j];
     }
-    // p_hat = min(p_i, p_j : type_j != 2, |r|^2 <= R0^2)
-    __device__ __forceinline__ void visit_min(const Cand<Real> c)
-    {
-        const Real rr = rr3(c.x - xi, c.y - yi, c.z - zi);
-        const bool near = (c.t != 2) & (rr <= re2) & (c.a < ph);
-        ph = near ? c.a : ph;
-    }
-    __device__ __forceinline__ void visit_grad(const Cand<Real> c)
+    __device__ __forceinline__ void visit(const Cand<Real> c)
     {
         const Real dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
         const Real rr = rr3(dx, dy, dz);
+        const bool near = (c.t != 2) & (rr <= re2) & (c.a < ph);
+        ph = near ? c.a : ph;
         const bool inside = (rr < tn) & (c.j != i) & !(skip_dummy & (c.t == 2));
         const Real ri = rsq(rr);
-        const Real f = inside ? (c.a - ph) * (ri * ri) * fmar(re, ri, (Real)-1) : (Real)0;
-        gx = fmar(f, dx, gx);
-        gy = fmar(f, dy, gy);
-        gz = fmar(f, dz, gz);
+        const Real s = inside ? (ri * ri) * fmar(re, ri, (Real)-1) : (Real)0;
+        const Real f = s * (c.a - pi);
+        ax = fmar(f, dx, ax);
+        ay = fmar(f, dy, ay);
+        az = fmar(f, dz, az);
+        bx = fmar(s, dx, bx);
+        by = fmar(s, dy, by);
+        bz = fmar(s, dz, bz);
     }
     __device__ void process(const Cand<Real>* buf, int cnt)
     {
-        if (pass == 0) {
-            for_each_cand<4>(buf, cnt, [&](const Cand<Real>& c, int) { visit_min(c); });
-        } else {
-            for_each_cand<4>(buf, cnt, [&](const Cand<Real>& c, int) { visit_grad(c); });
-        }
+        for_each_cand<4>(buf, cnt, [&](const Cand<Real>& c, int) { visit(c); });
     }
 };
 
@@ -342,33 +343,25 @@ __global__ void __launch_bounds__(64) k_gradp(Grid g, const int* __restrict__ cs
         const bool in = i < s1;
         op.i = i;
         op.active = false;
-        op.pass = 0;
-        op.gx = op.gy = op.gz = (Real)0;
+        op.ax = op.ay = op.az = op.bx = op.by = op.bz = (Real)0;
         Real ui = 0, vi = 0, wi = 0;
         if (in) {
             op.xi = a.x[i]; op.yi = a.y[i]; op.zi = a.z[i];
             ui = a.u[i]; vi = a.v[i]; wi = a.w[i];
-            op.ph = a.p[i];
+            op.pi = a.p[i];
+            op.ph = op.pi;
             op.active = a.type[i] == 0;
         }
         Real lo[3], hi[3];
         wave_box(op.active, op.xi, op.yi, op.zi, lo, hi);
-        if (ballot(op.active)) {
-            StageResult r = stage<CAP>(g, cs, cell, lo, hi, R2, buf, op);
-            op.pass = 1;
-            if (!r.overflowed) {
-                __syncthreads();
-                op.process(buf, r.count);
-            } else {
-                __syncthreads();
-                stage<CAP>(g, cs, cell, lo, hi, R2, buf, op);
-            }
-        }
+        if (ballot(op.active)) stage<CAP>(g, cs, cell, lo, hi, R2, buf, op);
         if (in) {
             Real xo = op.xi, yo = op.yi, zo = op.zi, uo = ui, vo = vi, wo = wi;
             if (op.active) {
+                const Real dp = op.pi - op.ph;
+                const Real gx = fmar(dp, op.bx, op.ax), gy = fmar(dp, op.by, op.ay), gz = fmar(dp, op.bz, op.az);
                 const Real s = (Real)(-dt / kc.rho * (double)kc.dim / n0);
-                const Real du = s * op.gx, dv = s * op.gy, dw = s * op.gz;
+                const Real du = s * gx, dv = s * gy, dw = s * gz;
                 const Real dtr = (Real)dt;
                 uo = ui + du; vo = vi + dv; wo = wi + dw;
                 xo = op.xi + dtr * du; yo = op.yi + dtr * dv; zo = op.zi + dtr * dw;
@@ -382,7 +375,6 @@ __global__ void __launch_bounds__(64) k_gradp(Grid g, const int* __restrict__ cs
     umax2 = wave_max(umax2);
     if (threadIdx.x == 0 && umax2 > 0.0f) atomicMax(&sc->maxu2_bits, __float_as_uint(umax2));
 }
-
 
 // LDS batches of 128 FP32 candidates (4 KiB; 64 FP64 candidates): VGPRs, not
 // LDS, then bound the resident wavefronts (8 per SIMD).  256- and 512-candidate
