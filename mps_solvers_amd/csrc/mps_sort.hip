@@ -79,8 +79,8 @@ __global__ void __launch_bounds__(RS_THREADS) k_radix_hist(const uint32_t* __res
     hist[(size_t)tid * gridDim.x + blockIdx.x] = h[tid];
 }
 
-// Exclusive scan of m counters in three launches: per-tile totals, a scan of
-// the tile totals by one block, then each tile rescanned with its offset.
+// Exclusive scan of m counters in two launches: per-tile totals, then each
+// tile rescanned with its offset.
 constexpr int SC_THREADS = 256;
 constexpr int SC_ITEMS = 16;
 constexpr int SC_TILE = SC_THREADS * SC_ITEMS;
@@ -121,23 +121,16 @@ __global__ void __launch_bounds__(SC_THREADS) k_scan_tiles(const uint32_t* __res
     if (threadIdx.x == 0) tiles[blockIdx.x] = tot;
 }
 
-__global__ void __launch_bounds__(SC_THREADS) k_scan_top(uint32_t* __restrict__ tiles, int nt)
-{
-    uint32_t carry = 0;
-    for (int base = 0; base < nt; base += SC_THREADS) {
-        const int i = base + threadIdx.x;
-        const uint32_t v = i < nt ? tiles[i] : 0u;
-        uint32_t tot;
-        const uint32_t ex = block_excl_scan(v, &tot);
-        if (i < nt) tiles[i] = carry + ex;
-        carry += tot;
-    }
-}
-
-// each thread owns SC_ITEMS consecutive counters of the tile
+// Each thread owns SC_ITEMS consecutive counters of the tile.  The tile's
+// offset is the sum of the totals of the tiles before it, which every block
+// forms itself (a few thousand tiles at most: no separate scan of the totals).
 __global__ void __launch_bounds__(SC_THREADS) k_scan_apply(uint32_t* __restrict__ a, int m,
                                                            const uint32_t* __restrict__ tiles)
 {
+    uint32_t before = 0;
+    for (int t = threadIdx.x; t < (int)blockIdx.x; t += SC_THREADS) before += tiles[t];
+    uint32_t carry;
+    block_excl_scan(before, &carry);
     const int base = blockIdx.x * SC_TILE + threadIdx.x * SC_ITEMS;
     uint32_t v[SC_ITEMS];
     uint32_t s = 0;
@@ -147,7 +140,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_scan_apply(uint32_t* __restrict_
         s += v[k];
     }
     uint32_t tot;
-    uint32_t run = tiles[blockIdx.x] + block_excl_scan(s, &tot);
+    uint32_t run = carry + block_excl_scan(s, &tot);
 #pragma unroll
     for (int k = 0; k < SC_ITEMS; k++) {
         if (base + k < m) a[base + k] = run;
@@ -271,7 +264,6 @@ static void scan_excl(uint32_t* a, int m, uint32_t* tiles, hipStream_t st)
 {
     const int nt = cdiv(m, SC_TILE);
     hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(SC_THREADS), 0, st, a, m, tiles);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SC_THREADS), 0, st, tiles, nt);
     hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(SC_THREADS), 0, st, a, m, tiles);
 }
 

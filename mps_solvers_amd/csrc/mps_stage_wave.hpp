@@ -44,6 +44,13 @@ __device__ __forceinline__ Real box_d2(Real qx, Real qy, Real qz, const Real* lo
 
 struct StageResult { int count; bool overflowed; };
 
+// Distance along one axis from the box [lo, hi] to the slab [a, b] (0 if they overlap).
+template <typename Real>
+__device__ __forceinline__ Real axis_gap(Real lo, Real hi, Real a, Real b)
+{
+    return fmax(fmax(a - hi, lo - b), (Real)0);
+}
+
 // Visit buf[0..cnt) in ascending order.  Candidates are read from LDS U at a
 // time before the first of them is visited, so that the LDS latency of a
 // group overlaps the arithmetic of the group before it (a plain loop waits
@@ -66,25 +73,55 @@ __device__ __forceinline__ void for_each_cand(const Cd* buf, int cnt, F&& f)
 
 // Stream the stencil of `cell`, keep candidates within sqrt(R2) of the box,
 // and hand full LDS batches (CAP candidates of type Cd) to op.process().  The
-// last batch stays in `buf`.
-template <int CAP, typename Real, class Op, class Cd>
+// last batch stays in `buf`.  With CULL, stencil cells whose extent (widened
+// by 1e-4 h, far beyond the rounding of the hash) lies at least sqrt(R2) from
+// the box are not read: none of their particles could pass the box filter.
+// Only the collision radius (0.8 l0 against 4 l0 cells) leaves such cells;
+// the larger radii would pay the extra per-row arithmetic for nothing.
+template <int CAP, bool CULL = false, typename Real, class Op, class Cd>
 __device__ StageResult stage(const Grid& g, const int* __restrict__ cs, int cell, const Real* lo,
                              const Real* hi, Real R2, Cd* buf, Op& op)
 {
     const int lane = threadIdx.x;
     int cx, cy, cz;
     cell_xyz(g, cell, cx, cy, cz);
-    const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < g.dx ? cx + 1 : g.dx - 1;
+    const Real h = (Real)g.h, mg = (Real)1e-4 * h;
+    const Real left = (Real)g.left, back = (Real)g.back, bottom = (Real)g.bottom;
+    int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < g.dx ? cx + 1 : g.dx - 1;
+    if (CULL) {
+        const Real gl = axis_gap(lo[0], hi[0], left + (Real)x0 * h - mg, left + (Real)(x0 + 1) * h + mg);
+        const Real gr = axis_gap(lo[0], hi[0], left + (Real)x1 * h - mg, left + (Real)(x1 + 1) * h + mg);
+        // the x-end cells of every stencil row are at least this far away
+        if (x0 < cx && gl * gl >= R2) x0 = cx;
+        if (x1 > cx && gr * gr >= R2) x1 = cx;
+    }
     int count = 0;
     bool over = false;
     for (int dz = -1; dz <= 1; dz++) {
         const int zz = cz + dz;
         if (zz < 0 || zz >= g.dz) continue;
+        const Real gz = CULL ? axis_gap(lo[2], hi[2], bottom + (Real)zz * h - mg, bottom + (Real)(zz + 1) * h + mg)
+                             : (Real)0;
+        if (CULL && gz * gz >= R2) continue;
         for (int dy = -1; dy <= 1; dy++) {
             const int yy = cy + dy;
             if (yy < 0 || yy >= g.dy) continue;
+            int r0 = x0, r1 = x1;
+            if (CULL) {
+                const Real gy = axis_gap(lo[1], hi[1], back + (Real)yy * h - mg, back + (Real)(yy + 1) * h + mg);
+                const Real gyz = gz * gz + gy * gy;
+                if (gyz >= R2) continue;
+                if (r0 < cx) {
+                    const Real gx = axis_gap(lo[0], hi[0], left + (Real)r0 * h - mg, left + (Real)(r0 + 1) * h + mg);
+                    if (gyz + gx * gx >= R2) r0 = cx;
+                }
+                if (r1 > cx) {
+                    const Real gx = axis_gap(lo[0], hi[0], left + (Real)r1 * h - mg, left + (Real)(r1 + 1) * h + mg);
+                    if (gyz + gx * gx >= R2) r1 = cx;
+                }
+            }
             const int row = zz * g.sheet + yy * g.dx;
-            const int a = cs[row + x0], b = cs[row + x1 + 1];
+            const int a = cs[row + r0], b = cs[row + r1 + 1];
             for (int base = a; base < b; base += WAVE) {
                 const int j = base + lane;
                 Cd c;

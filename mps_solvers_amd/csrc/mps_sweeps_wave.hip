@@ -262,7 +262,7 @@ __global__ void __launch_bounds__(64) k_collision(Grid g, const int* __restrict_
         Real lo[3], hi[3];
         wave_box(op.active, op.xi, op.yi, op.zi, lo, hi);
         if (ballot(op.active)) {
-            stage<CAP>(g, cs, cell, lo, hi, R2, buf, op);
+            stage<CAP, true>(g, cs, cell, lo, hi, R2, buf, op);
         }
         if (in) {
             Real cx = op.active ? op.cx : (Real)0, cy = op.active ? op.cy : (Real)0,
