@@ -182,12 +182,12 @@ struct OpFill {
         int m = 0;
         for (int k0 = 0; k0 < cnt; k0 += 16) {
             wv::for_each_cand<8>(buf + k0, min(16, cnt - k0), [&](const wv::CandP<Real>& c, int k) {
+                // branch-free append (m <= FILL_LMAX - 1 at every write)
                 const Real rr = rr3(c.x - xi, c.y - yi, c.z - zi);
                 const bool self = (c.j & PPE_JM) == i;
-                if (active && (self || ((rr < tl) & !(c.j & PPE_T2)))) {
-                    L[m * WAVE + threadIdx.x] = (uint16_t)(self ? FILL_DIAG : k0 + k);
-                    m++;
-                }
+                const bool take = active & (self | ((rr < tl) & !(c.j & PPE_T2)));
+                L[m * WAVE + threadIdx.x] = (uint16_t)(self ? FILL_DIAG : k0 + k);
+                m += take ? 1 : 0;
             });
             // at most 16 appends between checks: no lane passes FILL_LMAX
             if (ballot(m > FILL_LMAX - 16)) {

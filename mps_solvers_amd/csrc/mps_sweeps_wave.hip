@@ -59,11 +59,12 @@ struct OpPnd {
     }
     __device__ __forceinline__ void test(const CandP<Real> c, int k, int& m)
     {
+        // branch-free append: the slot is written either way (m <= PND_LMAX - 1
+        // here, see process) and only a kept candidate advances m
         const Real rr = rr3(c.x - xi, c.y - yi, c.z - zi);
-        if (active && rr < tn && c.j != i) {
-            L[m * WAVE + threadIdx.x] = (uint16_t)k;
-            m++;
-        }
+        const bool take = active & (rr < tn) & (c.j != i);
+        L[m * WAVE + threadIdx.x] = (uint16_t)k;
+        m += take ? 1 : 0;
     }
     __device__ void process(const CandP<Real>* buf, int cnt)
     {
