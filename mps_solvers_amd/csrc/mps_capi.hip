@@ -567,9 +567,12 @@ extern "C" int mps_create(const mps_params* p, int np, int nb2, int nb1, const d
     if (!rc) rc = dmalloc(c, (void**)&c->cs, 4 * ((size_t)c->g.ncells + 1));
     if (!rc) rc = dmalloc(c, (void**)&c->sc, sizeof(DevScalars));
     if (rc) { mps_destroy(c); return rc; }
+    // every remaining set-up call is checked: a context is returned whole or not at all
+    bool good = true;
+    auto ok = [&good](hipError_t e) { good = good && e == hipSuccess; };
     for (int t = 0; t < T_N; t++) {
-        hipEventCreate(&c->ev[t][0]);
-        hipEventCreate(&c->ev[t][1]);
+        ok(hipEventCreate(&c->ev[t][0]));
+        ok(hipEventCreate(&c->ev[t][1]));
     }
 
     // state: types by index range (MPS.cpp:263-268), id = i, no normals yet
@@ -580,12 +583,12 @@ extern "C" int mps_create(const mps_params* p, int np, int nb2, int nb1, const d
         return rc;
     }
     if (!vel)
-        for (int q = 3; q < 6; q++) hipMemset(c->f[0][q], 0, rs * N);
-    if (!press) hipMemset(c->f[0][6], 0, rs * N);
-    hipMemcpy(c->type[0], ht.data(), 4 * N, hipMemcpyHostToDevice);
-    hipMemcpy(c->id[0], hid.data(), 4 * N, hipMemcpyHostToDevice);
-    hipMemcpy(c->normal[0], hn.data(), 4 * N, hipMemcpyHostToDevice);
-    hipMemset(c->n[0], 0, 8 * N);
+        for (int q = 3; q < 6; q++) ok(hipMemset(c->f[0][q], 0, rs * N));
+    if (!press) ok(hipMemset(c->f[0][6], 0, rs * N));
+    ok(hipMemcpy(c->type[0], ht.data(), 4 * N, hipMemcpyHostToDevice));
+    ok(hipMemcpy(c->id[0], hid.data(), 4 * N, hipMemcpyHostToDevice));
+    ok(hipMemcpy(c->normal[0], hn.data(), 4 * N, hipMemcpyHostToDevice));
+    ok(hipMemset(c->n[0], 0, 8 * N));
     DevScalars s0{};
     s0.dt = p->dt_max;
     s0.maxu2_bits = 0;
@@ -594,7 +597,12 @@ extern "C" int mps_create(const mps_params* p, int np, int nb2, int nb1, const d
         memcpy(&s0.maxu2_bits, &f2, 4);
     }
     s0.argmax = 0x7fffffff;
-    hipMemcpy(c->sc, &s0, sizeof s0, hipMemcpyHostToDevice);
+    ok(hipMemcpy(c->sc, &s0, sizeof s0, hipMemcpyHostToDevice));
+    if (!good) {
+        fprintf(stderr, "mps_amd: mps_create: device set-up failed\n");
+        mps_destroy(c);
+        return MPS_ERR_HIP;
+    }
 
     // MPS_GPU::Initial order (MPS_GPU.cpp:100-106): divideCell, makeLink, calOnce
     if ((rc = divide(c)) || (rc = c->prec ? calonce_t<double>(c) : calonce_t<float>(c))) {

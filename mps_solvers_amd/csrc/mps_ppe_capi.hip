@@ -37,6 +37,7 @@ struct PpeBuffers {
     void *e = nullptr, *r = nullptr, *p = nullptr, *q = nullptr;
     double *partial = nullptr, *partial2 = nullptr;
     PpeScalars* ps = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;   // solve timing of mps_step_semi
     bool assembled = false;
 };
 
@@ -53,6 +54,8 @@ void mps_ppe_free(mps_ctx* c)
     void* ptrs[] = {B->rp, B->col, B->v32, B->v64, B->b, B->x, B->x0_ext, B->bc, B->e, B->r, B->p,
                     B->q, B->partial, B->partial2, B->ps};
     for (void* p : ptrs) ffree(p);
+    if (B->t0) hipEventDestroy(B->t0);
+    if (B->t1) hipEventDestroy(B->t1);
     delete B;
     c->ppe = nullptr;
 }
@@ -81,6 +84,8 @@ static int ensure_rows(mps_ctx* c, int n)
     if (!B->ps) {
         HIPOK(hipMalloc(&B->ps, sizeof(PpeScalars)));
         HIPOK(hipMemset(B->ps, 0, sizeof(PpeScalars)));
+        HIPOK(hipEventCreate(&B->t0));
+        HIPOK(hipEventCreate(&B->t1));
     }
     HIPOK(hipMemset(B->x0_ext, 0, 8 * N));
     B->nmax = n;
@@ -338,19 +343,15 @@ extern "C" int mps_step_semi(mps_ctx* c, int nsteps, double alpha, int x0_kind, 
         } else if (rec && (rc = mps_mlp_prepare(c))) {
             break;
         }
-        hipEvent_t e0, e1;
-        hipEventCreate(&e0);
-        hipEventCreate(&e1);
-        hipEventRecord(e0, c->st);
+        PpeBuffers* B = c->ppe;
+        HIPOK(hipEventRecord(B->t0, c->st));
         int it = 0;
         double rel = 0.0;
         rc = mps_ppe_solve(c, 1, x0_kind == 2 ? 1 : x0_kind, nullptr, tol, 100000, &it, &rel);
-        hipEventRecord(e1, c->st);
-        hipEventSynchronize(e1);
+        HIPOK(hipEventRecord(B->t1, c->st));
+        HIPOK(hipEventSynchronize(B->t1));
         float t = 0.0f;
-        hipEventElapsedTime(&t, e0, e1);
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
+        HIPOK(hipEventElapsedTime(&t, B->t0, B->t1));
         ms += t;
         total += it;
         if (rc) break;
