@@ -10,6 +10,8 @@
 // Errors are returned as the C ABI's status codes instead of exit().
 #pragma once
 
+#include <chrono>
+#include <cstdarg>
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -22,7 +24,11 @@ class MPS_GPU_EXPL {
 public:
     explicit MPS_GPU_EXPL(int precision = 0, int rebin_every_move = 0)
         : precision_(precision), rebin_(rebin_every_move) {}
-    virtual ~MPS_GPU_EXPL() { mps_destroy(ctx_); }
+    virtual ~MPS_GPU_EXPL()
+    {
+        mps_destroy(ctx_);
+        if (log_) fclose(log_);
+    }
     MPS_GPU_EXPL(const MPS_GPU_EXPL&) = delete;
     MPS_GPU_EXPL& operator=(const MPS_GPU_EXPL&) = delete;
 
@@ -30,8 +36,16 @@ public:
     int Initial(const std::string& dir = ".")
     {
         dir_ = dir;
+        if (log_) fclose(log_);
+        log_ = fopen((dir + "/out/LOG.txt").c_str(), "wt");   // the run log, as the reference keeps it
         int rc = mps_load_case((dir + "/input.dat").c_str(), &P_);
-        if (rc) return rc;
+        if (rc) {
+            note("input.dat: could not be read (status %d)\n", rc);
+            return rc;
+        }
+        note("input.dat: dim %d, dp %g m, R0 %g dp, RLAP %g dp, rho %g kg/m3, nu %g m2/s, c %g m/s, "
+             "dt_max %g s, TT %g s, TOUT %g s\n",
+             P_.dim, P_.dp, P_.re_n_mult, P_.re_lap_mult, P_.rho, P_.nu, P_.cs, P_.dt_max, P_.tt, P_.tout);
         P_.precision = precision_;
         P_.rebin_every_move = rebin_;
         int np = 0, nb2 = 0, nb1 = 0;
@@ -44,7 +58,17 @@ public:
             return rc;
         mps_destroy(ctx_);
         ctx_ = nullptr;
-        return mps_create(&P_, np, nb2, nb1, pos.data(), vel.data(), pr.data(), &ctx_);
+        rc = mps_create(&P_, np, nb2, nb1, pos.data(), vel.data(), pr.data(), &ctx_);
+        if (rc) {
+            note("GEO.dat: %d particles; set-up failed with status %d\n", np, rc);
+            return rc;
+        }
+        double g[4];
+        int d[4];
+        mps_get_grid(ctx_, g, d);
+        note("GEO.dat: %d particles (%d dummy, %d wall, %d fluid); %d x %d x %d cells of %g m\n", np, nb2, nb1,
+             np - nb2 - nb1, d[0], d[1], d[2], g[3]);
+        return MPS_OK;
     }
 
     // MPS_GPU_EXPL::step (MPS_CAL_expl_v1.0/MPS_GPU_EXPL.cpp:85-147)
@@ -65,6 +89,10 @@ public:
         int count = 0;
         double threshold = 0.0;
         char name[64];
+        const auto t0 = std::chrono::steady_clock::now();
+        auto wall = [&t0]() {
+            return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        };
         while (d_time_ < P_.tt && (max_steps < 0 || i_step_ < max_steps)) {
             if (d_time_ - threshold >= 0.0) {
                 snprintf(name, sizeof name, "/out/%04d.out", count);
@@ -74,12 +102,18 @@ public:
                     snprintf(name, sizeof name, "/out/part%04d.vtk", count);
                     if ((rc = writeVtk(dir_ + name))) return rc;
                 }
+                note("output %04d: step %ld, t = %.6f s, dt = %.3e s, wall %.3f s%s\n", count, i_step_, d_time_,
+                     d_dt_, wall(), stepNote().c_str());
                 threshold += P_.tout;
                 count++;
             }
             int rc = step();
-            if (rc) return rc;
+            if (rc) {
+                note("step %ld stopped with status %d (t = %.6f s)\n", i_step_, rc, d_time_);
+                return rc;
+            }
         }
+        note("done: %ld steps, t = %.6f s, wall %.3f s\n", i_step_, d_time_, wall());
         return MPS_OK;
     }
 
@@ -121,9 +155,28 @@ protected:
     void advance()
     {
         double s[10];
-        if (mps_get_scalars(ctx_, s) == MPS_OK) d_time_ = s[4];
+        if (mps_get_scalars(ctx_, s) == MPS_OK) {
+            d_time_ = s[4];
+            d_dt_ = s[3];
+        }
         i_step_++;
         stale_ = true;
+    }
+    // extra text for the output lines of the log (the semi-implicit step adds its CG figures)
+    virtual std::string stepNote() const { return std::string(); }
+    // one line to stdout and to out/LOG.txt (when the out folder exists)
+    void note(const char* fmt, ...)
+    {
+        va_list a;
+        va_start(a, fmt);
+        vprintf(fmt, a);
+        va_end(a);
+        if (log_) {
+            va_start(a, fmt);
+            vfprintf(log_, fmt, a);
+            va_end(a);
+            fflush(log_);
+        }
     }
     void refresh()
     {
@@ -139,8 +192,9 @@ protected:
     std::string dir_ = ".";
     mps_params P_{};
     mps_ctx* ctx_ = nullptr;
-    double d_time_ = 0.0;
+    double d_time_ = 0.0, d_dt_ = 0.0;
     long i_step_ = 0;
+    FILE* log_ = nullptr;
     bool stale_ = true;
     bool vtk_ = false;
     std::vector<double> pos_, vel_, pr_;
@@ -169,6 +223,14 @@ public:
     }
     int lastIterations() const { return last_iters_; }
     double lastSolveMs() const { return last_solve_ms_; }
+
+protected:
+    std::string stepNote() const override
+    {
+        char b[96];
+        snprintf(b, sizeof b, ", last CG %d iterations in %.3f ms", last_iters_, last_solve_ms_);
+        return b;
+    }
 
 private:
     double tol_;

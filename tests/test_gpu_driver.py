@@ -37,3 +37,10 @@ def test_driver_runs_reference_files(tmp_path, mode):
     rec = np.array(tok, float).reshape(-1, 9)
     assert sorted(rec[:, 0].astype(int).tolist()) == list(range(len(pos)))
     assert set(rec[:, 1].astype(int).tolist()) <= {0, 1, 2}
+    # the run log (the reference's out/LOG.txt): inputs, geometry, one line per output, the end
+    log = open(tmp_path / "out" / "LOG.txt").read().splitlines()
+    assert log[0].startswith("input.dat: dim 3") and log[1].startswith(f"GEO.dat: {len(pos)} particles")
+    assert sum(line.startswith("output ") for line in log) == len(outs)
+    assert log[-1].startswith("done: ")
+    if "--semi" in mode:
+        assert "CG" in log[2]
