@@ -34,6 +34,7 @@ constexpr int NIN = 16, NH = 64;
 constexpr int OW1 = 0, OB1 = 1024, OW2 = 1088, OB2 = 5184, OW3 = 5248, OB3 = 9344, OW4 = 9408, OB4 = 9472;
 constexpr int NPARAM = 9473;
 constexpr int MT = 256;   // threads per MLP block (4 waves, 16 samples per wave per pass)
+constexpr int INFER_BLOCKS = 512;
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -370,6 +371,7 @@ __global__ void __launch_bounds__(256) k_mlp_reduce_adam(int nb, int B, const fl
 // Pass A (16 lanes per row): residual of the previous pressure r = b - A x,
 // diagonal D, off-diagonal weight sum W, Jacobi correction c = r / D.
 constexpr int FG = 16;
+constexpr int FU = 4;   // entries per lane in flight in the feature passes
 
 __global__ void __launch_bounds__(256) k_feat_a(int n, const int* __restrict__ rp, const int* __restrict__ col,
                                                 const double* __restrict__ val, const double* __restrict__ b,
@@ -382,11 +384,25 @@ __global__ void __launch_bounds__(256) k_feat_a(int n, const int* __restrict__ r
     if (row >= n) return;
     const int k0 = rp[row], k1 = rp[row + 1];
     double ax = 0.0, d = 0.0, w = 0.0;
-    for (int k = k0 + g; k < k1; k += FG) {
-        const int j = col[k];
-        const double a = val[k];
-        ax += a * x[j];
-        if (j == row) d += a; else w += a;
+    // FU entries per lane loaded in one predicated batch before their gathers
+    // (as the CG's SpMV), added in ascending order
+    for (int k = k0 + g; k < k1; k += FU * FG) {
+        int jj[FU];
+        double av[FU], xv[FU];
+#pragma unroll
+        for (int u = 0; u < FU; u++) {
+            const int kk = k + u * FG;
+            jj[u] = kk < k1 ? col[kk] : -1;
+            av[u] = kk < k1 ? val[kk] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < FU; u++) xv[u] = jj[u] >= 0 ? x[jj[u]] : 0.0;
+#pragma unroll
+        for (int u = 0; u < FU; u++) {
+            if (jj[u] < 0) continue;
+            ax += av[u] * xv[u];
+            if (jj[u] == row) d += av[u]; else w += av[u];
+        }
     }
 #pragma unroll
     for (int m = FG / 2; m >= 1; m >>= 1) {
@@ -421,18 +437,35 @@ __global__ void __launch_bounds__(256) k_feat_b(int n, const int* __restrict__ r
     const double n0 = sc->n0;
     const double xi = x[row], ci = cvec[row], zi = (double)z[row];
     double sc_ = 0, sx = 0, sdev = 0, sxz = 0, szz = 0, scz = 0, sxx = 0;
-    for (int k = k0 + g; k < k1; k += FG) {
-        const int j = col[k];
-        if (j == row) continue;
-        const double a = val[k];
-        const double dz = (double)z[j] - zi, dx = x[j] - xi, dc = cvec[j] - ci;
-        sc_ += a * cvec[j];
-        sx += a * x[j];
-        sdev += a * (nn[j] - n0) / n0;
-        sxz += a * dx * dz;
-        szz += a * dz * dz;
-        scz += a * dc * dz;
-        sxx += a * dx * dx;
+    for (int k = k0 + g; k < k1; k += FU * FG) {
+        int jj[FU];
+        double av[FU];
+#pragma unroll
+        for (int u = 0; u < FU; u++) {
+            const int kk = k + u * FG;
+            const int j = kk < k1 ? col[kk] : row;
+            jj[u] = j == row ? -1 : j;   // the diagonal and the slots past the row add nothing
+            av[u] = kk < k1 ? val[kk] : 0.0;
+        }
+        double xj[FU], cj[FU], nj[FU], zj[FU];
+#pragma unroll
+        for (int u = 0; u < FU; u++) {
+            const int j = jj[u] >= 0 ? jj[u] : row;
+            xj[u] = x[j]; cj[u] = cvec[j]; nj[u] = nn[j]; zj[u] = (double)z[j];
+        }
+#pragma unroll
+        for (int u = 0; u < FU; u++) {
+            if (jj[u] < 0) continue;
+            const double a = av[u];
+            const double dz = zj[u] - zi, dx = xj[u] - xi, dc = cj[u] - ci;
+            sc_ += a * cj[u];
+            sx += a * xj[u];
+            sdev += a * (nj[u] - n0) / n0;
+            sxz += a * dx * dz;
+            szz += a * dz * dz;
+            scz += a * dc * dz;
+            sxx += a * dx * dx;
+        }
     }
 #pragma unroll
     for (int m = FG / 2; m >= 1; m >>= 1) {
@@ -781,7 +814,9 @@ int mps_mlp_guess(mps_ctx* c)
     MlpBuffers* M = c->mlp;
     double* out = mps_ppe_guess_buffer(c, M->nrows);
     if (!out) return MPS_ERR_HIP;
-    const int grid = std::min(4096, (M->nrows + 63) / 64);
+    // a few blocks per CU: each block loads the weights into LDS once and then
+    // streams its rows (the load is the fixed cost of a block)
+    const int grid = std::min(INFER_BLOCKS, (M->nrows + 63) / 64);
     hipLaunchKernelGGL(k_mlp_infer, dim3(std::max(grid, 1)), dim3(MT), 0, c->st, M->nrows, M->F, M->th, M->ident,
                        M->xprev, mlp_ps(c), nullptr, out);
     HIPOK(hipGetLastError());
