@@ -108,19 +108,21 @@ def test_semi_implicit_steps_run_and_converge(precision):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-def test_semi_implicit_trajectory_matches_oracle(precision):
+@pytest.mark.parametrize("rebin", [0, 1])
+def test_semi_implicit_trajectory_matches_oracle(precision, rebin):
     """5 semi-implicit steps (alpha = 1/(rho c^2)) on the device against the
     oracle's semi-implicit simulator (FP64 CG to the same relative residual),
-    by particle id.  The two solvers stop at different points inside the 1e-8
+    by particle id, with one re-binning per step or one after every position
+    update.  The two solvers stop at different points inside the 1e-8
     residual ball, so the bars are looser than the explicit step's: FP64 path
     p and u within 1e-5 normwise, FP32 path within 2e-3."""
     nb2, nb1, pos = small_dam(5, jitter=0.0)
     kw = dict(re_lap_mult=3.1, dt_max=DT)
     alpha = 1.0 / (1000.0 * 20.0 ** 2)
-    s = mps.Solver(gpu_params(precision, **kw), nb2, nb1, pos)
+    s = mps.Solver(gpu_params(precision, rebin_every_move=rebin, **kw), nb2, nb1, pos)
     sim = oracle.Sim(oracle_params(precision, **kw), nb2, nb1, pos[:, 0], pos[:, 1], pos[:, 2])
     s.step_semi(5, alpha=alpha, x0_kind=1, tol=1e-8)
-    sim.step_semi(5, alpha=alpha, tol=1e-8)
+    sim.step_semi(5, alpha=alpha, tol=1e-8, rebin_every_move=bool(rebin))
     g = s.state(by_id=True)
     o = sim.state()
     order = np.argsort(o["id"])
