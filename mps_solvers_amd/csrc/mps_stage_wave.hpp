@@ -164,5 +164,128 @@ __device__ __forceinline__ void wave_box(bool act, Real x, Real y, Real z, Real*
 template <typename Real>
 __device__ __forceinline__ Real filt_r2(Real R) { Real r = R * (Real)1.00001 + (Real)1e-12; return r * r; }
 
+// ------------------------------------------------------------ lane groups ---
+// A chunk of up to 64 particles is cut into ng <= G groups of at most
+// S = 64 / G particles, and each group stages and walks its own candidate set
+// with G lanes per particle: lane q*S + s takes the staged candidates
+// q, q + G, q + 2G, ... (phase q) for the group's s-th particle, and the G
+// partial sums are combined at the end.  A group's box is a fraction of the
+// chunk's, so each particle evaluates fewer candidates; the price is one
+// staging pass per group and the final combine.
+
+// Far-away candidate used to pad a batch to a multiple of G: every cutoff
+// test fails on it and it carries zero values, so it adds exactly nothing.
+template <typename Real>
+__device__ __forceinline__ void far_cand(CandP<Real>& c)
+{
+    c.x = c.y = c.z = (Real)1e30;
+    c.j = -1;
+}
+template <typename Real>
+__device__ __forceinline__ void far_cand(Cand<Real>& c)
+{
+    c.x = c.y = c.z = (Real)1e30;
+    c.j = -1;
+    c.t = 0;
+    c.a = c.b = c.c = (Real)0;
+}
+
+// Pad buf[0..cnt) with far candidates to a multiple of G; returns the
+// candidates per phase.  buf must hold cnt + G - 1 entries.
+template <int G, class Cd>
+__device__ __forceinline__ int pad_phases(Cd* buf, int cnt)
+{
+    if (G == 1) return cnt;
+    const int padded = (cnt + G - 1) / G * G;
+    if ((int)threadIdx.x < padded - cnt) {
+        Cd c;
+        far_cand(c);
+        buf[cnt + threadIdx.x] = c;
+    }
+    __syncthreads();
+    return padded / G;
+}
+
+// Visit phase q's candidates buf[k*G + q], k in [k0, k1), in ascending order,
+// reading U of them from LDS before the first is visited (as for_each_cand).
+template <int U, int G, class Cd, class F>
+__device__ __forceinline__ void for_each_phase(const Cd* buf, int k0, int k1, int q, F&& f)
+{
+    int k = k0;
+    for (; k + U <= k1; k += U) {
+        Cd c[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) c[u] = buf[(k + u) * G + q];
+#pragma unroll
+        for (int u = 0; u < U; u++) f(c[u], (k + u) * G + q);
+    }
+    for (; k < k1; k++) f(buf[k * G + q], k * G + q);
+}
+
+// Combine the G phase lanes of each particle (lanes s, s + S, s + 2S, ...).
+template <int G, typename T>
+__device__ __forceinline__ T phase_sum(T v)
+{
+#pragma unroll
+    for (int m = WAVE / G; m < WAVE; m <<= 1) v += __shfl_xor(v, m);
+    return v;
+}
+template <int G, typename T>
+__device__ __forceinline__ T phase_min(T v)
+{
+#pragma unroll
+    for (int m = WAVE / G; m < WAVE; m <<= 1) { T o = __shfl_xor(v, m); v = o < v ? o : v; }
+    return v;
+}
+
+// Cut the active lanes into groups: ng = the smallest power of two <= G with
+// ng * S >= (active lanes); each level halves every group at the median of a
+// 6-bit quantised coordinate along the longest remaining axis of the chunk's
+// box (ties by lane), so group sizes differ by at most one and none exceeds S.
+// On return ord[g * S + s] is the lane of group g's s-th particle (ascending
+// lanes within a group) or -1.  ord holds 64 ints.
+template <int G, typename Real>
+__device__ int split_groups(bool act, Real x, Real y, Real z, const Real* lo, const Real* hi, int* ord)
+{
+    constexpr int S = WAVE / G;
+    const int lane = threadIdx.x;
+    const uint64_t am = ballot(act);
+    const int n = __popcll(am);
+    int ng = 1;
+    while (ng < G && ng * S < n) ng <<= 1;
+    ord[lane] = -1;
+    const uint64_t below = lane ? (~0ull >> (WAVE - lane)) : 0ull;
+    uint64_t mine = am;
+    int grp = 0;
+    Real ext[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+    for (int lv = 1; lv < ng; lv <<= 1) {
+        int a = ext[1] > ext[0] ? 1 : 0;
+        a = ext[2] > ext[a] ? 2 : a;
+        const Real e = ext[a];
+        ext[a] = e * (Real)0.5;
+        const Real c = a == 0 ? x : (a == 1 ? y : z);
+        int key = 0;
+        if (e > (Real)0) key = min(63, max(0, (int)((c - lo[a]) * ((Real)64 / e))));
+        uint64_t eq = mine;
+        int less = 0;
+#pragma unroll
+        for (int b = 5; b >= 0; b--) {
+            const bool bit = (key >> b) & 1;
+            const uint64_t m1 = ballot(bit);
+            if (bit) less += __popcll(eq & ~m1);
+            eq &= bit ? m1 : ~m1;
+        }
+        less += __popcll(eq & below);
+        const bool up = less >= (__popcll(mine) >> 1);
+        const uint64_t mu = ballot(act && up);
+        mine &= up ? mu : ~mu;
+        grp = 2 * grp + (up ? 1 : 0);
+    }
+    __syncthreads();
+    if (act) ord[grp * S + __popcll(mine & below)] = lane;
+    __syncthreads();
+    return ng;
+}
+
 }  // namespace wv
 }  // namespace mps
