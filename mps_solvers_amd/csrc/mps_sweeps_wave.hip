@@ -172,6 +172,7 @@ __global__ void __launch_bounds__(64) k_visc(Grid g, const int* __restrict__ cs,
     constexpr int S = WAVE / G;
     __shared__ Cand<Real> buf[CAP + G];
     __shared__ int ord[WAVE];
+    __shared__ Real sums[4][WAVE];   // per chunk particle: sum w, sum w (u_j - ub)
     const int cell = xcd_cell(g.ncells);
     if (cell >= g.ncells) return;
     const int s0 = cs[cell], s1 = cs[cell + 1];
@@ -216,19 +217,27 @@ __global__ void __launch_bounds__(64) k_visc(Grid g, const int* __restrict__ cs,
             const Real sw = phase_sum<G>(op.sw);
             const Real lx = phase_sum<G>(op.lx), ly = phase_sum<G>(op.ly), lz = phase_sum<G>(op.lz);
             if (has && op.q == 0) {
-                const Real uj = a.u[j], vj = a.v[j], wj = a.w[j];
-                const Real dtr = (Real)dt, nc = (Real)(kc.nu * coef);
-                const Real tx = fmar(-(uj - op.ub), sw, lx);
-                const Real ty = fmar(-(vj - op.vb), sw, ly);
-                const Real tz = fmar(-(wj - op.wb), sw, lz);
-                const Real uo = uj + dtr * (nc * tx + (Real)kc.g[0]);
-                const Real vo = vj + dtr * (nc * ty + (Real)kc.g[1]);
-                const Real wo = wj + dtr * (nc * tz + (Real)kc.g[2]);
-                b.x[j] = op.xi + dtr * uo; b.y[j] = op.yi + dtr * vo; b.z[j] = op.zi + dtr * wo;
-                b.u[j] = uo; b.v[j] = vo; b.w[j] = wo;
+                sums[0][o] = sw; sums[1][o] = lx; sums[2][o] = ly; sums[3][o] = lz;
             }
             __syncthreads();
         }
+        // each particle's own lane finishes it: one coalesced store per array
+        // (its state is read again rather than held in registers across the groups)
+        if (act) {
+            const Real sw = sums[0][lane], lx = sums[1][lane], ly = sums[2][lane], lz = sums[3][lane];
+            xi = a.x[i]; yi = a.y[i]; zi = a.z[i];
+            ui = a.u[i]; vi = a.v[i]; wi = a.w[i];
+            const Real dtr = (Real)dt, nc = (Real)(kc.nu * coef);
+            const Real tx = fmar(-(ui - op.ub), sw, lx);
+            const Real ty = fmar(-(vi - op.vb), sw, ly);
+            const Real tz = fmar(-(wi - op.wb), sw, lz);
+            const Real uo = ui + dtr * (nc * tx + (Real)kc.g[0]);
+            const Real vo = vi + dtr * (nc * ty + (Real)kc.g[1]);
+            const Real wo = wi + dtr * (nc * tz + (Real)kc.g[2]);
+            b.x[i] = xi + dtr * uo; b.y[i] = yi + dtr * vo; b.z[i] = zi + dtr * wo;
+            b.u[i] = uo; b.v[i] = vo; b.w[i] = wo;
+        }
+        __syncthreads();
     }
 }
 

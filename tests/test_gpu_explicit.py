@@ -188,6 +188,59 @@ def test_pressure_gradient_operator(precision):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("lap", [2.5, 5.0])
+def test_sweeps_at_other_cell_sizes(precision, lap):
+    """The cell edge is max(re_n, re_lap).  At re_lap = 5 l0 a cell holds up to
+    ~125 particles, so a cell is swept in two chunks (64 and the rest); at
+    2.5 l0 it holds at most 27, so the viscosity sweep keeps each chunk in one
+    lane group.  Viscosity, collision, PND + p and the pressure gradient, each
+    against the oracle from an identical state."""
+    kw = dict(nu=0.05, re_lap_mult=lap, col_mult=1.0)
+
+    def state():
+        s, P = _solver(precision, **kw)
+        _perturb(s, precision, 0.15)
+        _, cs = s.stage_sort()
+        s.set_dt(1e-4)
+        return s, P, cs, s.state()
+
+    s, P, cs, st = state()
+    occ = np.diff(cs)
+    assert (occ.max() > 64) if lap == 5.0 else (occ.max() <= 32), occ.max()
+    o = oracle_view(st)
+    ref = oracle.visc(P, oracle_grid(s), o, cs, s.scalars()["coef"], 1e-4)
+    s.stage_visc()
+    out = s.state()
+    for a, k in enumerate("uvw"):
+        assert normwise(out["vel"][:, a] - o[k], ref[k] - o[k]) < TOL[precision], ("visc", k)
+    _pos_ok(out["pos"], o, ref, precision)
+
+    s, P, cs, st = state()
+    o = oracle_view(st)
+    ref, ncol = oracle.collision(P, oracle_grid(s), o, cs, 1e-4)
+    s.stage_collision()
+    assert ncol > 0 and int(s.scalars()["ncol"]) == ncol
+    out = s.state()
+    for a, k in enumerate("uvw"):
+        assert normwise(out["vel"][:, a] - o[k], ref[k] - o[k]) < TOL[precision], ("collision", k)
+
+    s, P, cs, st = state()
+    o = oracle_view(st)
+    n_o = oracle.pnd(P, oracle_grid(s), o["x"], o["y"], o["z"], o["type"], cs, n_init=o["n"])
+    s.stage_pnd_pres()
+    assert normwise(s.state()["n"], n_o) < 1e-12
+
+    s, P, cs, st = state()
+    o = oracle_view(st)
+    ref = oracle.gradp(P, oracle_grid(s), o, o["p"], cs, s.scalars()["n0"], 1e-4)
+    s.stage_gradp()
+    out = s.state()
+    for a, k in enumerate("uvw"):
+        assert normwise(out["vel"][:, a] - o[k], ref[k] - o[k]) < TOL[precision], ("gradp", k)
+    _pos_ok(out["pos"], o, ref, precision)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
 def test_short_trajectory(precision):
     """20 full steps on the device against the oracle simulator, by particle id."""
     nb2, nb1, pos = small_dam(7)
