@@ -400,7 +400,8 @@ static int run_stage_t(mps_ctx* c, Stage s)
         break;
     case S_COL:
         tic(c, T_COL);
-        hipMemsetAsync(&c->sc->ncol, 0, sizeof(int), c->st);
+        // the hit counter is cleared by the dt kernel at the start of every step
+        // (and by mps_stage_collision when the stage runs on its own)
         launch_collision<Real>(c->g, c->cs, cur, alt, thr<Real>(c), c->kc, c->sc, c->st);
         c->kin ^= 1;
         toc(c, T_COL);
@@ -853,7 +854,12 @@ static int stage_sync(mps_ctx* c, Stage s)
 }
 
 extern "C" int mps_stage_visc(mps_ctx* c) { return stage_sync(c, S_VISC); }
-extern "C" int mps_stage_collision(mps_ctx* c) { return stage_sync(c, S_COL); }
+extern "C" int mps_stage_collision(mps_ctx* c)
+{
+    if (!c) return MPS_ERR_ARG;
+    HIPOK(hipMemsetAsync(&c->sc->ncol, 0, sizeof(int), c->st));
+    return stage_sync(c, S_COL);
+}
 extern "C" int mps_stage_pnd_pres(mps_ctx* c)
 {
     int rc = stage_sync(c, S_PND);
