@@ -360,20 +360,34 @@ Thresh<double> make_thresh<double>(const mps_params& P)
     return t;
 }
 
+// Re-binning (MPS_GPU::divideCell + sort_*).  The first sort of a state the
+// host supplied runs the radix sort; every later one starts from the previous
+// sort's order and gathers each new cell from the 27 old cells around it.
+// Both give the stable permutation of the current order by cell key.
 template <typename Real>
 static int divide_t(mps_ctx* c)
 {
     StateView<Real> cur = c->view<Real>(c->kin, c->aux);
     StateView<Real> dst = c->view<Real>(1 - c->kin, 1 - c->aux);
-    launch_hash<Real>(c->g, c->nt, cur, c->k0, c->v0, &c->sc->nexcl, c->st);
-    uint32_t *kr, *vr;
-    launch_radix_sort(c->nt, c->key_bits, c->k0, c->v0, c->k1, c->v1, c->hist, &kr, &vr, c->st);
-    launch_cell_bounds(kr, c->nt, c->g.ncells, c->cs, c->st);
-    launch_reorder<Real>(c->nt, vr, cur, dst, c->inv, c->ntmp, c->st);
+    const bool gather = c->binned;
+    launch_hash<Real>(c->g, c->nt, cur, c->k0, c->v0, &c->sc->nexcl, gather ? (uint32_t*)c->cs_next : nullptr,
+                      c->st);
+    uint32_t* vr;
+    if (gather) {
+        launch_gather_sort(c->g, c->nt, c->k0, c->cs, (uint32_t*)c->cs_next, c->cs_tiles, c->v1, c->st);
+        vr = c->v1;
+    } else {
+        uint32_t* kr;
+        launch_radix_sort(c->nt, c->key_bits, c->k0, c->v0, c->k1, c->v1, c->hist, &kr, &vr, c->st);
+        launch_cell_bounds(kr, c->nt, c->g.ncells, c->cs_next, c->st);
+    }
+    std::swap(c->cs, c->cs_next);
+    launch_reorder<Real>(c->nt, vr, cur, dst, c->inv, c->ntmp, c->cs_next, c->g.ncells + 1, c->st);
     launch_tail_copy<Real>(c->nt, c->cs + c->g.ncells, dst, cur, c->st);
     c->perm = vr;
     c->kin ^= 1;
     c->aux ^= 1;
+    c->binned = true;
     return hipGetLastError() == hipSuccess ? MPS_OK : MPS_ERR_HIP;
 }
 
@@ -488,6 +502,8 @@ template <typename Real>
 static int upload_t(mps_ctx* c, int k, int a, const double* pos, const double* vel, const double* press)
 {
     const int N = c->nt;
+    // host-supplied positions can be anywhere: the next re-binning sorts from scratch
+    if (pos) c->binned = false;
     std::vector<Real> h((size_t)N);
     for (int q = 0; q < 3; q++) {
         if (pos) {
@@ -566,6 +582,8 @@ extern "C" int mps_create(const mps_params* p, int np, int nb2, int nb1, const d
     if (!rc) rc = dmalloc(c, (void**)&c->inv, 4 * N);
     if (!rc) rc = dmalloc(c, (void**)&c->ntmp, 4 * N);
     if (!rc) rc = dmalloc(c, (void**)&c->cs, 4 * ((size_t)c->g.ncells + 1));
+    if (!rc) rc = dmalloc(c, (void**)&c->cs_next, 4 * ((size_t)c->g.ncells + 1));
+    if (!rc) rc = dmalloc(c, (void**)&c->cs_tiles, 4 * (size_t)scan_tile_words(c->g.ncells + 1));
     if (!rc) rc = dmalloc(c, (void**)&c->sc, sizeof(DevScalars));
     if (rc) { mps_destroy(c); return rc; }
     // every remaining set-up call is checked: a context is returned whole or not at all
@@ -590,6 +608,7 @@ extern "C" int mps_create(const mps_params* p, int np, int nb2, int nb1, const d
     ok(hipMemcpy(c->id[0], hid.data(), 4 * N, hipMemcpyHostToDevice));
     ok(hipMemcpy(c->normal[0], hn.data(), 4 * N, hipMemcpyHostToDevice));
     ok(hipMemset(c->n[0], 0, 8 * N));
+    ok(hipMemset(c->cs_next, 0, 4 * ((size_t)c->g.ncells + 1)));
     DevScalars s0{};
     s0.dt = p->dt_max;
     s0.maxu2_bits = 0;

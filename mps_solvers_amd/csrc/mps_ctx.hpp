@@ -35,6 +35,12 @@ struct mps_ctx {
     uint32_t *k0 = nullptr, *v0 = nullptr, *k1 = nullptr, *v1 = nullptr, *hist = nullptr;
     uint32_t* perm = nullptr;   // last sort's permutation (one of v0/v1)
     int *inv = nullptr, *ntmp = nullptr, *cs = nullptr;
+    // the other cell-start array (all zeros between sorts: the next re-binning
+    // counts keys into it), the tile totals of its scan, and whether the
+    // current order is a previous sort's (re-bin by gathering) or not (radix sort)
+    int* cs_next = nullptr;
+    uint32_t* cs_tiles = nullptr;
+    bool binned = false;
     mps::DevScalars* sc = nullptr;
     long steps = 0;
     bool timing = false;

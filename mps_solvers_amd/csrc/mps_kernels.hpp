@@ -57,15 +57,23 @@ void launch_gradp(const Grid& g, const int* cs, StateView<Real> a, StateView<Rea
 
 // sort
 int sort_scratch_words(int n);
+// hash (and, with cnt, per-cell key counts: cnt[0..ncells] must hold zeros)
 template <typename Real>
 void launch_hash(const Grid& g, int n, StateView<Real> s, uint32_t* key, uint32_t* val, int* nexcl,
-                 hipStream_t st);
+                 uint32_t* cnt, hipStream_t st);
 void launch_radix_sort(int n, int key_bits, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
                        uint32_t* hist, uint32_t** kres, uint32_t** vres, hipStream_t st);
 void launch_cell_bounds(const uint32_t* keys, int n, int ncells, int* cs, hipStream_t st);
+// stable re-binning of a previously sorted order: cs_new holds the hash's
+// counts on entry and the new cell starts on return; perm receives the
+// permutation; tiles needs scan_tile_words(ncells + 1) words
+int scan_tile_words(int m);
+void launch_gather_sort(const Grid& g, int n, const uint32_t* key, const int* cs_old, uint32_t* cs_new,
+                        uint32_t* tiles, uint32_t* perm, hipStream_t st);
+// gather the state into sorted order; also clears zero[0..nzero)
 template <typename Real>
 void launch_reorder(int n, const uint32_t* perm, StateView<Real> src, StateView<Real> dst, int* inv,
-                    int* normal_tmp, hipStream_t st);
+                    int* normal_tmp, int* zero, int nzero, hipStream_t st);
 // copy positions and velocities of the excluded slots [cs[ncells], n) from one
 // state buffer into the other, so that both hold them until the next sort
 template <typename Real>

@@ -16,34 +16,123 @@ namespace mps {
 // c_a = integer((pos_a - origin_a) / h), truncation toward zero, taken in the
 // state precision (MPS_CAL_expl_v1.0/MPS_GPU.cpp:265-268).  Out-of-grid
 // particles become type 3 with zero velocity (MPS_GPU.cpp:270-280) and get
-// key = ncells so that they sort after every cell.
+// key = ncells so that they sort after every cell.  With cnt, the keys are
+// also counted per cell (cnt[ncells]: the excluded ones): keys arrive nearly
+// sorted, so each run of equal keys in a wavefront adds its length once.
 template <typename Real>
 __global__ void __launch_bounds__(256) k_hash(Grid g, int n, const Real* __restrict__ x,
                                               const Real* __restrict__ y, const Real* __restrict__ z,
                                               int* __restrict__ type, Real* u, Real* v, Real* w,
                                               uint32_t* __restrict__ key, uint32_t* __restrict__ val,
-                                              int* __restrict__ nexcl)
+                                              int* __restrict__ nexcl, uint32_t* __restrict__ cnt)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    val[i] = (uint32_t)i;
-    int t = type[i];
-    if (t == 3) { key[i] = (uint32_t)g.ncells; return; }
-    const Real left = (Real)g.left, back = (Real)g.back, bottom = (Real)g.bottom, h = (Real)g.h;
-    Real fx = (x[i] - left) / h;
-    Real fy = (y[i] - back) / h;
-    Real fz = (z[i] - bottom) / h;
-    Real tx = trunc(fx), ty = trunc(fy), tz = trunc(fz);
-    bool in = tx >= (Real)0 && ty >= (Real)0 && tz >= (Real)0 && tx < (Real)g.dx &&
-              ty < (Real)g.dy && tz < (Real)g.dz;
-    if (!in) {
-        key[i] = (uint32_t)g.ncells;
-        type[i] = 3;
-        u[i] = (Real)0; v[i] = (Real)0; w[i] = (Real)0;
-        atomicAdd(nexcl, 1);
-        return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    uint32_t kk = 0xffffffffu;
+    if (valid) {
+        val[i] = (uint32_t)i;
+        kk = (uint32_t)g.ncells;
+        if (type[i] != 3) {
+            const Real left = (Real)g.left, back = (Real)g.back, bottom = (Real)g.bottom, h = (Real)g.h;
+            Real fx = (x[i] - left) / h;
+            Real fy = (y[i] - back) / h;
+            Real fz = (z[i] - bottom) / h;
+            Real tx = trunc(fx), ty = trunc(fy), tz = trunc(fz);
+            bool in = tx >= (Real)0 && ty >= (Real)0 && tz >= (Real)0 && tx < (Real)g.dx &&
+                      ty < (Real)g.dy && tz < (Real)g.dz;
+            if (in) {
+                kk = (uint32_t)((int)tz * g.sheet + (int)ty * g.dx + (int)tx);
+            } else {
+                type[i] = 3;
+                u[i] = (Real)0; v[i] = (Real)0; w[i] = (Real)0;
+                atomicAdd(nexcl, 1);
+            }
+        }
+        key[i] = kk;
     }
-    key[i] = (uint32_t)((int)tz * g.sheet + (int)ty * g.dx + (int)tx);
+    if (cnt) {
+        const int lane = threadIdx.x & 63;
+        const uint32_t prev = __shfl_up(kk, 1);
+        const bool head = valid && (lane == 0 || prev != kk);
+        const uint64_t bound = ballot(head || !valid);
+        if (head) {
+            const uint64_t up = lane == 63 ? 0ull : (bound >> (lane + 1));
+            const int next = up ? lane + 1 + __builtin_ctzll(up) : 64;
+            atomicAdd(&cnt[kk], (uint32_t)(next - lane));
+        }
+    }
+}
+
+// ------------------------------------------------- re-binning a running state ---
+// Between two sorts of a running simulation a particle moves a fraction of a
+// cell, so in the current (previously sorted) order the particles of new
+// cell k lie in the 27 cells around k.  Walking those old cells in ascending
+// index walks their slots in ascending order, so collecting the particles
+// whose new key is k gives exactly the stable order within k.  One wavefront
+// per new cell does that, writing into the cell's range of the scanned
+// counts.  A cell that collects fewer particles than it counted (one came from
+// further away, or left the grid) walks every slot instead: the result is the
+// stable permutation in every case, only slower for such a cell.
+__global__ void __launch_bounds__(64) k_gather_cells(Grid g, int n, const uint32_t* __restrict__ key,
+                                                     const int* __restrict__ cs_old,
+                                                     const uint32_t* __restrict__ cs_new,
+                                                     uint32_t* __restrict__ perm)
+{
+    const int k = xcd_cell(g.ncells + 1);
+    if (k > g.ncells) return;
+    const int lane = threadIdx.x;
+    const int lo = (int)cs_new[k], hi = k < g.ncells ? (int)cs_new[k + 1] : n;
+    const int cnt = hi - lo;
+    if (cnt <= 0) return;
+    int found = 0;
+    // slots [a, b) in ascending order, four 64-slot chunks loaded before the
+    // first is matched (the loads are independent: one latency per four chunks)
+    auto collect = [&](int a, int b) {
+        for (int base = a; base < b; base += 4 * WAVE) {
+            uint32_t kk[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int j = base + u * WAVE + lane;
+                kk[u] = j < b ? key[j] : 0xffffffffu;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int j = base + u * WAVE + lane;
+                const bool hit = kk[u] == (uint32_t)k;
+                const uint64_t m = ballot(hit);
+                const int pos = found + lanes_below(m);
+                if (hit && pos < cnt) perm[lo + pos] = (uint32_t)j;
+                found += __popcll(m);
+            }
+        }
+    };
+    if (k < g.ncells) {
+        int cx, cy, cz;
+        cell_xyz(g, k, cx, cy, cz);
+        const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < g.dx ? cx + 1 : g.dx - 1;
+        // lane r < 9 reads the slot range of stencil row r (dz-major, as the walk)
+        int ra = 0, rb = 0;
+        if (lane < 9) {
+            const int zz = cz + lane / 3 - 1, yy = cy + lane % 3 - 1;
+            if (zz >= 0 && zz < g.dz && yy >= 0 && yy < g.dy) {
+                const int row = zz * g.sheet + yy * g.dx;
+                ra = cs_old[row + x0];
+                rb = cs_old[row + x1 + 1];
+            }
+        }
+        for (int r = 0; r < 9; r++) collect(__shfl(ra, r), __shfl(rb, r));
+    } else {
+        // the excluded tail: unchanged unless particles left the grid since
+        const int a = cs_old[g.ncells];
+        if (n - a == cnt) {
+            for (int r = lane; r < cnt; r += WAVE) perm[lo + r] = (uint32_t)(a + r);
+            found = cnt;
+        }
+    }
+    if (found != cnt) {
+        found = 0;
+        collect(0, n);
+    }
 }
 
 // ------------------------------------------------------------ radix sort ---
@@ -214,9 +303,12 @@ __global__ void __launch_bounds__(256) k_cell_bounds(const uint32_t* __restrict_
 template <typename Real>
 __global__ void __launch_bounds__(256) k_reorder(int n, const uint32_t* __restrict__ perm,
                                                  StateView<Real> src, StateView<Real> dst,
-                                                 int* __restrict__ inv, int* __restrict__ normal_tmp)
+                                                 int* __restrict__ inv, int* __restrict__ normal_tmp,
+                                                 int* __restrict__ zero, int nzero)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
+    // clear the retired cell-start array: it counts the next re-binning's keys
+    for (int c = k; c < nzero; c += gridDim.x * blockDim.x) zero[c] = 0;
     if (k >= n) return;
     int o = (int)perm[k];
     dst.x[k] = src.x[o]; dst.y[k] = src.y[o]; dst.z[k] = src.z[o];
@@ -269,11 +361,22 @@ static void scan_excl(uint32_t* a, int m, uint32_t* tiles, hipStream_t st)
 
 template <typename Real>
 void launch_hash(const Grid& g, int n, StateView<Real> s, uint32_t* key, uint32_t* val, int* nexcl,
-                 hipStream_t st)
+                 uint32_t* cnt, hipStream_t st)
 {
     if (n == 0) return;
     hipLaunchKernelGGL(k_hash<Real>, dim3(cdiv(n, 256)), dim3(256), 0, st, g, n, s.x, s.y, s.z,
-                       s.type, s.u, s.v, s.w, key, val, nexcl);
+                       s.type, s.u, s.v, s.w, key, val, nexcl, cnt);
+}
+
+int scan_tile_words(int m) { return cdiv(m, SC_TILE) + 64; }
+
+void launch_gather_sort(const Grid& g, int n, const uint32_t* key, const int* cs_old, uint32_t* cs_new,
+                        uint32_t* tiles, uint32_t* perm, hipStream_t st)
+{
+    if (n == 0) return;
+    scan_excl(cs_new, g.ncells + 1, tiles, st);
+    hipLaunchKernelGGL(k_gather_cells, dim3(xcd_grid(g.ncells + 1)), dim3(64), 0, st, g, n, key, cs_old,
+                       (const uint32_t*)cs_new, perm);
 }
 
 void launch_radix_sort(int n, int key_bits, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1,
@@ -302,11 +405,11 @@ void launch_cell_bounds(const uint32_t* keys, int n, int ncells, int* cs, hipStr
 
 template <typename Real>
 void launch_reorder(int n, const uint32_t* perm, StateView<Real> src, StateView<Real> dst, int* inv,
-                    int* normal_tmp, hipStream_t st)
+                    int* normal_tmp, int* zero, int nzero, hipStream_t st)
 {
     if (n == 0) return;
     hipLaunchKernelGGL(k_reorder<Real>, dim3(cdiv(n, 256)), dim3(256), 0, st, n, perm, src, dst, inv,
-                       normal_tmp);
+                       normal_tmp, zero, nzero);
     hipLaunchKernelGGL(k_normal_remap, dim3(cdiv(n, 256)), dim3(256), 0, st, n, normal_tmp, inv,
                        dst.normal);
 }
@@ -320,9 +423,13 @@ void launch_tail_copy(int n, const int* np_active, StateView<Real> from, StateVi
 
 template void launch_tail_copy<float>(int, const int*, StateView<float>, StateView<float>, hipStream_t);
 template void launch_tail_copy<double>(int, const int*, StateView<double>, StateView<double>, hipStream_t);
-template void launch_hash<float>(const Grid&, int, StateView<float>, uint32_t*, uint32_t*, int*, hipStream_t);
-template void launch_hash<double>(const Grid&, int, StateView<double>, uint32_t*, uint32_t*, int*, hipStream_t);
-template void launch_reorder<float>(int, const uint32_t*, StateView<float>, StateView<float>, int*, int*, hipStream_t);
-template void launch_reorder<double>(int, const uint32_t*, StateView<double>, StateView<double>, int*, int*, hipStream_t);
+template void launch_hash<float>(const Grid&, int, StateView<float>, uint32_t*, uint32_t*, int*, uint32_t*,
+                                  hipStream_t);
+template void launch_hash<double>(const Grid&, int, StateView<double>, uint32_t*, uint32_t*, int*, uint32_t*,
+                                   hipStream_t);
+template void launch_reorder<float>(int, const uint32_t*, StateView<float>, StateView<float>, int*, int*, int*,
+                                    int, hipStream_t);
+template void launch_reorder<double>(int, const uint32_t*, StateView<double>, StateView<double>, int*, int*,
+                                     int*, int, hipStream_t);
 
 }  // namespace mps

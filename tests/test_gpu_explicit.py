@@ -71,6 +71,47 @@ def test_sort_permutation_bit_exact(precision):
 
 
 @pytest.mark.parametrize("precision", [0, 1])
+def test_rebinning_from_a_sorted_order(precision):
+    """Once sorted, a state is re-binned by gathering each new cell from the
+    27 old cells around it; a particle that jumps further, or leaves the grid,
+    makes its cell walk every slot.  Each case against the oracle's stable sort
+    of the same keys: ordinary moves (up to ~1 l0), then one particle carried
+    three cells in x and another out of the top of the grid."""
+    s, P = _solver(precision)
+    G = oracle_grid(s)
+    h = s.grid()["h"]
+
+    def check():
+        before = s.state()
+        key = oracle.hash_cells(P, G, before["pos"][:, 0], before["pos"][:, 1], before["pos"][:, 2],
+                                before["type"])
+        perm_o, _, cs_o = oracle.sort_cells(key, G.ncells)
+        perm, cs = s.stage_sort()
+        np.testing.assert_array_equal(perm, perm_o)
+        np.testing.assert_array_equal(cs, cs_o)
+        np.testing.assert_array_equal(s.state()["id"], before["id"][perm_o])
+        return perm
+
+    s.set_dt(0.03)    # 0.3 m/s random velocities: moves up to ~1 l0
+    s.stage_visc()
+    perm = check()
+    assert not np.array_equal(perm, np.arange(len(perm)))
+
+    st = s.state()
+    fl = np.nonzero(st["type"] == 0)[0]
+    vel = np.zeros_like(st["vel"])
+    dt = 1e-3
+    vel[fl[0], 0] = 3.2 * h / dt          # three cells along x within one update
+    vel[fl[len(fl) // 2], 2] = 60 * h / dt  # out of the top of the grid
+    s.set_state(vel=vel)
+    s.set_dt(dt)
+    nexcl = s.scalars()["nexcl"]
+    s.stage_visc()
+    check()
+    assert s.scalars()["nexcl"] == nexcl + 1
+
+
+@pytest.mark.parametrize("precision", [0, 1])
 def test_calonce_constants_and_normals(precision):
     s, P = _solver(precision)
     st = s.state()
