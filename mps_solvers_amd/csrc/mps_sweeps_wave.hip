@@ -345,18 +345,23 @@ struct OpGrad {
     Real xi, yi, zi, pi, ph;
     Real ax, ay, az, bx, by, bz;
 
+    // c.a = p_j; c.b = p_j as a p-hat candidate (the largest value for a dummy,
+    // which never lowers p-hat).  With skip_dummy a dummy adds nothing at all,
+    // so it is placed out of every box and never staged.
     __device__ void load_cand(int j, Cand<Real>& c)
     {
-        c.x = x[j]; c.y = y[j]; c.z = z[j]; c.j = j;
-        c.t = type[j]; c.a = p[j];
+        const bool dummy = type[j] == 2;
+        c.x = (skip_dummy && dummy) ? big<Real>() : x[j];
+        c.y = y[j]; c.z = z[j]; c.j = j;
+        c.a = p[j];
+        c.b = dummy ? big<Real>() : c.a;
     }
     __device__ __forceinline__ void visit(const Cand<Real> c)
     {
         const Real dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
         const Real rr = rr3(dx, dy, dz);
-        const bool near = (c.t != 2) & (rr <= re2) & (c.a < ph);
-        ph = near ? c.a : ph;
-        const bool inside = (rr < tn) & (c.j != i) & !(skip_dummy & (c.t == 2));
+        ph = rr <= re2 ? fmin(ph, c.b) : ph;
+        const bool inside = (rr < tn) & (c.j != i);
         const Real ri = rsq(rr);
         const Real s = inside ? (ri * ri) * fmar(re, ri, (Real)-1) : (Real)0;
         const Real f = s * (c.a - pi);
