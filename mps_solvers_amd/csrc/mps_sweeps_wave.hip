@@ -360,7 +360,7 @@ struct OpGrad {
     {
         const Real dx = c.x - xi, dy = c.y - yi, dz = c.z - zi;
         const Real rr = rr3(dx, dy, dz);
-        ph = rr <= re2 ? fmin(ph, c.b) : ph;
+        ph = (rr <= re2) & (c.b < ph) ? c.b : ph;
         const bool inside = (rr < tn) & (c.j != i);
         const Real ri = rsq(rr);
         const Real s = inside ? (ri * ri) * fmar(re, ri, (Real)-1) : (Real)0;
