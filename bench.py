@@ -289,8 +289,12 @@ def traffic_from_profile(dom, slug):
     return None
 
 
-# VALU issue peak: 157.3 TFLOP/s FP32 vector / (64 lanes x 2 flop per FMA)
+# VALU issue peak: 157.3 TFLOP/s FP32 vector / (64 lanes x 2 flop per FMA).  That
+# figure needs packed FP32 (two FMAs per lane per instruction); the kernels here are
+# built without packed FP32, so their ceiling is one wave64 instruction per SIMD
+# every 4 cycles: 256 CUs x 4 SIMDs x 2.4 GHz / 4.
 VALU_PEAK_INSTR_S = 157.3e12 / 128
+VALU_UNPACKED_INSTR_S = 256 * 4 * 2.4e9 / 4
 
 
 def valu_from_profile(dom, slug):
@@ -370,7 +374,8 @@ def main():
     if vi is not None:
         va = vi / (per[dom] * 1e-3)
         valu = {"instr_per_launch": vi, "achieved": va, "peak": VALU_PEAK_INSTR_S, "unit": "wave64 instr/s",
-                "frac": va / VALU_PEAK_INSTR_S}
+                "frac": va / VALU_PEAK_INSTR_S, "peak_unpacked": VALU_UNPACKED_INSTR_S,
+                "frac_unpacked": va / VALU_UNPACKED_INSTR_S}
     sc = s.scalars()
     s.close()
     # FP32 trajectory vs the FP64 device path (itself at ~1e-13 of the FP64 oracle in
