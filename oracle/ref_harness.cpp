@@ -3,11 +3,13 @@
 // vectors.  TEST INFRASTRUCTURE ONLY: it runs in the build container, never
 // on the GPU box, and neither the reference sources nor this binary ship.
 //
-// Usage: ref_v4 <workdir>
+// Usage: ref_v4 <workdir> [--time REPS]
 //   <workdir> holds input.dat and GEO.dat in the reference formats
 //   (MPS_CAL_v4.0/MPS.cpp:62-300).  The harness runs MPS_CPU::Initial
 //   (loadCase, createGeo, calOnce, divideCell), then the per-particle
 //   operators and the Poisson build + CG, and writes <workdir>/ref_out.bin.
+//   With --time it instead times the reference's explicit operators on that
+//   state (oracle/time_vs_ref.py compares them with the oracle's).
 //
 // The v4.0 tree is compiled with DIM 2 (MPS_CAL_v4.0/common.h:15), so its
 // divideCell<2> fills a 9-entry link table.  For 3-D inputs the harness
@@ -43,6 +45,44 @@ void put_i(const char* n, const std::vector<int>& v) { put(n, 1, (long)v.size(),
 
 class Harness : public MPS_CPU_EXPL {
 public:
+    int time_reps = 0;
+
+    // Wall time of the reference's own explicit operators on the loaded state:
+    // cal_n (its OpenMP loop) and the per-particle d3_LapVel / d3_GradPress,
+    // driven here by an OpenMP loop over particles (both only read the state).
+    void time_ops()
+    {
+        const int N = (int)i_np;
+        std::vector<double> sink(N);
+        auto ms = [](double t0) { return (omp_get_wtime() - t0) * 1e3; };
+        double t_n = 0, t_lap = 0, t_grad = 0;
+        for (int r = 0; r < time_reps; r++) {
+            double t0 = omp_get_wtime();
+            cal_n();
+            t_n += ms(t0);
+            t0 = omp_get_wtime();
+#pragma omp parallel for schedule(static)
+            for (int i = 0; i < N; i++) {
+                const real3 l = d3_LapVel(i);
+                sink[i] = l.x + l.y + l.z;
+            }
+            t_lap += ms(t0);
+            t0 = omp_get_wtime();
+#pragma omp parallel for schedule(static)
+            for (int i = 0; i < N; i++) {
+                const real3 g = d3_GradPress(i);
+                sink[i] += g.x + g.y + g.z;
+            }
+            t_grad += ms(t0);
+        }
+        double chk = 0;
+        for (int i = 0; i < N; i++) chk += sink[i];
+        std::printf("{\"n\": %d, \"threads\": %d, \"reps\": %d, \"cal_n_ms\": %.4f, \"lapvel_ms\": %.4f, "
+                    "\"gradpress_ms\": %.4f, \"checksum\": %.6e}\n",
+                    N, omp_get_max_threads(), time_reps, t_n / time_reps, t_lap / time_reps,
+                    t_grad / time_reps, chk);
+    }
+
     void run()
     {
         fid_log = std::fopen("./out/LOG.txt", "at");
@@ -60,6 +100,7 @@ public:
                 i_link_cell[c][0] = 27;
             }
         }
+        if (time_reps > 0) { time_ops(); return; }
         std::vector<double> buf(N);
         std::vector<int> ib(N);
 
@@ -134,13 +175,19 @@ public:
 
 int main(int argc, char** argv)
 {
-    if (argc < 2) { std::fprintf(stderr, "usage: ref_v4 <workdir>\n"); return 2; }
+    if (argc < 2) { std::fprintf(stderr, "usage: ref_v4 <workdir> [--time REPS]\n"); return 2; }
     if (chdir(argv[1]) != 0) { std::perror("chdir"); return 2; }
     mkdir("out", 0755);
+    Harness h;
+    if (argc >= 4 && !std::strcmp(argv[2], "--time")) {
+        // timing: the thread count comes from OMP_NUM_THREADS
+        h.time_reps = std::atoi(argv[3]);
+        h.run();
+        return 0;
+    }
     omp_set_num_threads(1);
     g_out = std::fopen("ref_out.bin", "wb");
     if (!g_out) return 2;
-    Harness h;
     h.run();
     std::fclose(g_out);
     return 0;
