@@ -120,14 +120,17 @@ __global__ void __launch_bounds__(64) k_ppe_count(Grid g, const int* __restrict_
     }
 }
 
-// Fill pass, two phases per LDS batch as in the PND sweep: the FP32 cutoff
-// test appends the row's diagonal and in-range non-dummy candidates to a
-// per-lane list (slot-major in LDS), then the FP64 weights run over the dense
-// lists and each lane writes its row's entries at consecutive CSR slots, in
+// Fill pass: 16 lanes per row, four rows at a time.  For each staged batch
+// and each group of four rows, lane p of a row tests staged candidates
+// p, p + 16, ..., forms the FP64 weight of the ones it keeps, and the row's
+// kept entries of each 16-candidate step are compacted by ballot into the
+// row's next CSR slots: every store instruction writes four contiguous runs
+// (one per row) instead of scattering one entry per lane over 64 rows, and
+// the lanes stay busy when a cell holds fewer than 64 particles.  Entries keep
 // the staged (ascending) order.  Dirichlet neighbours add to the diagonal but
-// get no entry.  The diagonal's value is written when the row is complete.
-constexpr int FILL_LMAX = 32;
-constexpr uint16_t FILL_DIAG = 0x8000u;
+// get no entry; the diagonal's value is written when the row is complete.
+// Per-row state (position, next slot, diagonal sum and slot) lives in LDS.
+constexpr int FILL_LANES = 16;
 
 template <typename Real>
 struct OpFill {
@@ -136,66 +139,71 @@ struct OpFill {
     const unsigned char* bcv;
     Real tl;
     double re;
-    int i;
-    bool active;
-    Real xi, yi, zi;
-    double diag;
-    long kpos, kdiag;
+    int nrow;
+    const Real *rx, *ry, *rz;
+    const int* ri;
+    const unsigned char* ract;
+    int *rkpos, *rkdiag;
+    double* rdiag;
     int* col;
     float* v32;
     double* v64;
-    uint16_t* L;   // L[s * WAVE + lane]
 
     __device__ void load_cand(int j, wv::CandP<Real>& c)
     {
         c.x = x[j]; c.y = y[j]; c.z = z[j];
         c.j = j | (type[j] == 2 ? PPE_T2 : 0) | (bcv[j] ? PPE_BC : 0);
     }
-    __device__ void drain(const wv::CandP<Real>* buf, int m)
-    {
-        const int lane = threadIdx.x;
-        const int mm = wave_max(m);
-        const double xd = (double)xi, yd = (double)yi, zd = (double)zi;
-        for (int s = 0; s < mm; s++) {
-            if (s < m) {
-                const uint16_t lk = L[s * WAVE + lane];
-                if (lk & FILL_DIAG) {
-                    kdiag = kpos;
-                    col[kpos++] = i;
-                } else {
-                    const wv::CandP<Real> c = buf[lk];
-                    const double dx = (double)c.x - xd, dy = (double)c.y - yd, dz = (double)c.z - zd;
-                    const double w = __builtin_fma(re, rsq(dx * dx + dy * dy + dz * dz), -1.0);
-                    diag -= w;
-                    if (!(c.j & PPE_BC)) {   // Dirichlet neighbour: A_ij = 0
-                        col[kpos] = c.j & PPE_JM;
-                        v64[kpos] = w;
-                        v32[kpos] = (float)w;
-                        kpos++;
-                    }
-                }
-            }
-        }
-    }
     __device__ void process(const wv::CandP<Real>* buf, int cnt)
     {
-        int m = 0;
-        for (int k0 = 0; k0 < cnt; k0 += 16) {
-            wv::for_each_cand<8>(buf + k0, min(16, cnt - k0), [&](const wv::CandP<Real>& c, int k) {
-                // branch-free append (m <= FILL_LMAX - 1 at every write)
+        const int lane = threadIdx.x, p = lane % FILL_LANES, r4 = lane / FILL_LANES;
+        const uint64_t seg = 0xffffull << (r4 * FILL_LANES);
+        const uint64_t below = lane ? (~0ull >> (WAVE - lane)) : 0ull;
+        for (int rg = 0; rg * (WAVE / FILL_LANES) < nrow; rg++) {
+            const int r = rg * (WAVE / FILL_LANES) + r4;
+            const bool rok = r < nrow && ract[r];
+            Real xi = 0, yi = 0, zi = 0;
+            int i = -1, kp = 0;
+            if (rok) { xi = rx[r]; yi = ry[r]; zi = rz[r]; i = ri[r]; kp = rkpos[r]; }
+            const double xd = (double)xi, yd = (double)yi, zd = (double)zi;
+            double dsum = 0.0;
+            int kd = -1;
+            for (int k0 = 0; k0 < cnt; k0 += FILL_LANES) {
+                const int k = k0 + p;
+                const bool kin = k < cnt;
+                const wv::CandP<Real> c = buf[kin ? k : 0];
                 const Real rr = rr3(c.x - xi, c.y - yi, c.z - zi);
                 const bool self = (c.j & PPE_JM) == i;
-                const bool take = active & (self | ((rr < tl) & !(c.j & PPE_T2)));
-                L[m * WAVE + threadIdx.x] = (uint16_t)(self ? FILL_DIAG : k0 + k);
-                m += take ? 1 : 0;
-            });
-            // at most 16 appends between checks: no lane passes FILL_LMAX
-            if (ballot(m > FILL_LMAX - 16)) {
-                drain(buf, m);
-                m = 0;
+                const bool take = rok & kin & (self | ((rr < tl) & !(c.j & PPE_T2)));
+                double w = 0.0;
+                if (take && !self) {
+                    const double dx = (double)c.x - xd, dy = (double)c.y - yd, dz = (double)c.z - zd;
+                    w = __builtin_fma(re, rsq(dx * dx + dy * dy + dz * dz), -1.0);
+                }
+                dsum += w;
+                const bool keep = take && (self || !(c.j & PPE_BC));   // Dirichlet neighbour: A_ij = 0
+                const uint64_t m = ballot(keep) & seg;
+                const int pos = kp + __popcll(m & below);
+                if (keep) {
+                    col[pos] = self ? i : (c.j & PPE_JM);
+                    v64[pos] = w;
+                    v32[pos] = (float)w;
+                }
+                kd = (keep && self) ? pos : kd;
+                kp += __popcll(m);
+            }
+#pragma unroll
+            for (int o = FILL_LANES / 2; o >= 1; o >>= 1) {
+                dsum += __shfl_xor(dsum, o);
+                const int t = __shfl_xor(kd, o);
+                kd = t > kd ? t : kd;
+            }
+            if (rok && p == 0) {
+                rkpos[r] = kp;
+                rdiag[r] -= dsum;
+                if (kd >= 0) rkdiag[r] = kd;
             }
         }
-        drain(buf, m);
     }
 };
 
@@ -207,7 +215,10 @@ __global__ void __launch_bounds__(64) k_ppe_fill(Grid g, const int* __restrict__
                                                  float* __restrict__ v32, double* __restrict__ v64)
 {
     __shared__ wv::CandP<Real> buf[CAP];
-    __shared__ uint16_t L[FILL_LMAX * WAVE];
+    __shared__ Real rx[WAVE], ry[WAVE], rz[WAVE];
+    __shared__ int ri[WAVE], rkpos[WAVE], rkdiag[WAVE];
+    __shared__ double rdiag[WAVE];
+    __shared__ unsigned char ract[WAVE];
     const int cell = xcd_cell(g.ncells);
     if (cell >= g.ncells) return;
     const int s0 = cs[cell], s1 = cs[cell + 1];
@@ -217,31 +228,39 @@ __global__ void __launch_bounds__(64) k_ppe_fill(Grid g, const int* __restrict__
     OpFill<Real> op;
     op.x = s.x; op.y = s.y; op.z = s.z; op.type = s.type; op.bcv = bc;
     op.tl = th.t_lap; op.re = (double)th.re_lap;
-    op.col = col; op.v32 = v32; op.v64 = v64; op.L = L;
+    op.rx = rx; op.ry = ry; op.rz = rz; op.ri = ri; op.ract = ract;
+    op.rkpos = rkpos; op.rkdiag = rkdiag; op.rdiag = rdiag;
+    op.col = col; op.v32 = v32; op.v64 = v64;
     const Real R2 = wv::filt_r2(th.re_lap);
+    const int lane = threadIdx.x;
     for (int c0 = s0; c0 < s1; c0 += WAVE) {
-        const int i = c0 + (int)threadIdx.x;
+        const int i = c0 + lane;
         const bool in = i < s1;
-        op.i = i;
-        op.active = false;
-        op.diag = 0.0;
-        op.kpos = in ? row_ptr[i] : 0;
-        op.kdiag = -1;
+        bool active = false;
+        Real xi = 0, yi = 0, zi = 0;
         if (in) {
-            op.xi = s.x[i]; op.yi = s.y[i]; op.zi = s.z[i];
-            op.active = !bc[i] && s.type[i] != 2;
+            xi = s.x[i]; yi = s.y[i]; zi = s.z[i];
+            active = !bc[i] && s.type[i] != 2;
         }
+        rx[lane] = xi; ry[lane] = yi; rz[lane] = zi; ri[lane] = i;
+        ract[lane] = active ? 1 : 0;
+        rkpos[lane] = in ? row_ptr[i] : 0;
+        rkdiag[lane] = -1;
+        rdiag[lane] = 0.0;
+        op.nrow = s1 - c0 < WAVE ? s1 - c0 : WAVE;
+        __syncthreads();
         Real lo[3], hi[3];
-        wv::wave_box(op.active, op.xi, op.yi, op.zi, lo, hi);
-        if (ballot(op.active)) wv::stage<CAP>(g, cs, cell, lo, hi, R2, buf, op);
+        wv::wave_box(active, xi, yi, zi, lo, hi);
+        if (ballot(active)) wv::stage<CAP>(g, cs, cell, lo, hi, R2, buf, op);
+        __syncthreads();
         if (in) {
-            if (!op.active) {
-                const long k = op.kpos;   // identity row
+            if (!active) {
+                const int k = row_ptr[i];   // identity row
                 col[k] = i; v64[k] = 1.0; v32[k] = 1.0f;
-            } else if (op.kdiag >= 0) {   // the row's own particle is always staged
-                const double d = op.diag - aii;
-                v64[op.kdiag] = d;
-                v32[op.kdiag] = (float)d;
+            } else if (rkdiag[lane] >= 0) {   // the row's own particle is always staged
+                const double d = rdiag[lane] - aii;
+                v64[rkdiag[lane]] = d;
+                v32[rkdiag[lane]] = (float)d;
             }
         }
         __syncthreads();
