@@ -107,7 +107,8 @@ int mps_sync(mps_ctx* c);
 int mps_get_np(mps_ctx* c, int* np_total, int* np_active);
 int mps_get_state(mps_ctx* c, int order, double* pos, double* vel, double* press, double* n,
                   int* id, int* type);
-/* Overwrite position / velocity / pressure in the current (sorted) order. */
+/* Overwrite position / velocity / pressure in the current (sorted) order.
+ * New positions may be anywhere, so the next re-binning sorts from scratch. */
 int mps_set_state(mps_ctx* c, const double* pos, const double* vel, const double* press);
 /* out[0..9] = n0, lambda, 2d/(n0 lambda), dt, time, ncells, collision hits of
  * the last collision sweep, excluded particles, cell size, step count. */
@@ -133,7 +134,10 @@ int mps_set_motion(mps_ctx* c, const unsigned char* moving, int axis, double amp
 /* ---------------------------------------------- per-stage entry points (parity) */
 
 /* MPS_GPU::divideCell + sort_* + sort_normal (MPS_CAL_expl_v1.0/MPS_GPU.cpp:248-373).
- * perm (np_total, new -> old) and cell_start (ncells + 1) may be NULL. */
+ * perm (np_total, new -> old) and cell_start (ncells + 1) may be NULL.  The
+ * permutation is the stable sort of the current order by cell, as the
+ * reference's, whichever way it is formed (a radix sort of a host-supplied
+ * state, or a gather from the previous cells of a state this library sorted). */
 int mps_stage_sort(mps_ctx* c, int* perm, int* cell_start);
 /* MPS_GPU_EXPL::calVisc_expl (MPS_CAL_expl_v1.0/MPS_GPU_EXPL.cpp:152-194) */
 int mps_stage_visc(mps_ctx* c);
