@@ -176,12 +176,6 @@ __device__ __forceinline__ Real filt_r2(Real R) { Real r = R * (Real)1.00001 + (
 // Far-away candidate used to pad a batch to a multiple of G: every cutoff
 // test fails on it and it carries zero values, so it adds exactly nothing.
 template <typename Real>
-__device__ __forceinline__ void far_cand(CandP<Real>& c)
-{
-    c.x = c.y = c.z = (Real)1e30;
-    c.j = -1;
-}
-template <typename Real>
 __device__ __forceinline__ void far_cand(Cand<Real>& c)
 {
     c.x = c.y = c.z = (Real)1e30;
@@ -228,13 +222,6 @@ __device__ __forceinline__ T phase_sum(T v)
 {
 #pragma unroll
     for (int m = WAVE / G; m < WAVE; m <<= 1) v += __shfl_xor(v, m);
-    return v;
-}
-template <int G, typename T>
-__device__ __forceinline__ T phase_min(T v)
-{
-#pragma unroll
-    for (int m = WAVE / G; m < WAVE; m <<= 1) { T o = __shfl_xor(v, m); v = o < v ? o : v; }
     return v;
 }
 
